@@ -1,0 +1,875 @@
+/*
+ * CPU oracle for the slater_stoch ZVMC two-electron-integral hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see zv_oracle.h).  Each function restates the
+ * reference routine named in its comment, keeping the reference's order of
+ * floating-point operations (left-to-right products, sequential sums, the
+ * FP32 casts of the estimator) so that:
+ *   - fixed-sample MC results match the HIP parity kernel bit for bit
+ *     (both use the exp of slater_stoch_amd/csrc/zv_math.h), and
+ *   - analytic Gaussian ERIs match the compiled reference to rounding.
+ *
+ * Build: oracle/Makefile (gcc -O2 -mfma -ffp-contract=off -fopenmp).
+ */
+#include "zv_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../slater_stoch_amd/csrc/zv_math.h"
+
+/* ------------------------------------------------------------------ */
+/* basis state (reference: module common_data, src/common_data.F90:36-56) */
+/* ------------------------------------------------------------------ */
+static int g_n = 0;
+static int g_ldg = 0;
+static double *g_center = NULL;  /* [n][3] */
+static int *g_npower = NULL;     /* [n][3] */
+static int *g_nsto = NULL;
+static int *g_itype = NULL;
+static double *g_zeta = NULL;
+static int *g_ngauss = NULL;
+static double *g_c1 = NULL; /* [n][ldg]  c_gauss(:,i,1) */
+static double *g_c2 = NULL; /* [n][ldg]  c_gauss(:,i,2) */
+static double *g_gg = NULL; /* [n][ldg]  g_gauss(:,i)   */
+static double g_azv = 0.0;
+
+int zvo_set_basis(int n, const double *center, const int *npower, const int *n_sto,
+                  const int *i_type, const double *zeta, const int *n_gauss, int ldg,
+                  const double *c_gauss, const double *g_gauss, double a_zv)
+{
+    if (n <= 0 || ldg <= 0) return 1;
+    free(g_center); free(g_npower); free(g_nsto); free(g_itype); free(g_zeta);
+    free(g_ngauss); free(g_c1); free(g_c2); free(g_gg);
+    g_n = n;
+    g_ldg = ldg;
+    g_center = malloc(sizeof(double) * 3 * n);
+    g_npower = malloc(sizeof(int) * 3 * n);
+    g_nsto = malloc(sizeof(int) * n);
+    g_itype = malloc(sizeof(int) * n);
+    g_zeta = malloc(sizeof(double) * n);
+    g_ngauss = malloc(sizeof(int) * n);
+    g_c1 = malloc(sizeof(double) * (size_t)ldg * n);
+    g_c2 = malloc(sizeof(double) * (size_t)ldg * n);
+    g_gg = malloc(sizeof(double) * (size_t)ldg * n);
+    memcpy(g_center, center, sizeof(double) * 3 * n);
+    memcpy(g_npower, npower, sizeof(int) * 3 * n);
+    memcpy(g_nsto, n_sto, sizeof(int) * n);
+    memcpy(g_itype, i_type, sizeof(int) * n);
+    memcpy(g_zeta, zeta, sizeof(double) * n);
+    memcpy(g_ngauss, n_gauss, sizeof(int) * n);
+    memcpy(g_c1, c_gauss, sizeof(double) * (size_t)ldg * n);
+    memcpy(g_c2, c_gauss + (size_t)ldg * n, sizeof(double) * (size_t)ldg * n);
+    memcpy(g_gg, g_gauss, sizeof(double) * (size_t)ldg * n);
+    g_azv = a_zv;
+    for (int i = 0; i < n; i++)
+        if (n_gauss[i] > ldg || n_gauss[i] < 1) return 2;
+    return 0;
+}
+
+int zvo_nbasis(void) { return g_n; }
+
+double zvo_exp(double x) { return zv_exp(x); }
+
+/* ------------------------------------------------------------------ */
+/* radial parts                                                        */
+/* ------------------------------------------------------------------ */
+
+/* src/u_orb.F90:1-26 (single STO, c_contract = 1 as set by
+ * src/read_basis.F90:36): r^n_STO exp(-zeta r) with the reference's
+ * multiplication order per case. */
+double zvo_u_orb(int i, double r)
+{
+    int o = i - 1;
+    double z = g_zeta[o];
+    double e = zv_exp(-z * r);
+    double r2;
+    switch (g_nsto[o]) {
+    case 0: return 1.0 * e;
+    case 1: return (1.0 * r) * e;
+    case 2: r2 = r * r; return (1.0 * r2) * e;
+    case 3: r2 = r * r; return ((1.0 * r2) * r) * e;
+    case 4: r2 = r * r; return ((1.0 * r2) * r2) * e;
+    default: return (1.0 * pow(r, g_nsto[o])) * e;
+    }
+}
+
+/* src/u_gauss.F90:1-13: sum_m (c1 + c2 r^2) exp(-g r^2), sequential. */
+double zvo_u_gauss(int i, double r)
+{
+    int o = i - 1;
+    const double *c1 = g_c1 + (size_t)g_ldg * o;
+    const double *c2 = g_c2 + (size_t)g_ldg * o;
+    const double *gg = g_gg + (size_t)g_ldg * o;
+    double r2 = r * r;
+    double u = 0.0;
+    for (int m = 0; m < g_ngauss[o]; m++) u = u + (c1[m] + c2[m] * r2) * zv_exp(-gg[m] * r2);
+    return u;
+}
+
+/* Cartesian prefactor of one AO at a point: src/compute_densities.F90:16-28 */
+static double poly_at(int o, double dx, double dy, double dz, double *rr)
+{
+    double p = 1.0;
+    for (int t = 0; t < g_npower[3 * o + 0]; t++) p = p * dx;
+    for (int t = 0; t < g_npower[3 * o + 1]; t++) p = p * dy;
+    for (int t = 0; t < g_npower[3 * o + 2]; t++) p = p * dz;
+    *rr = sqrt((dx * dx + dy * dy) + dz * dz);
+    return p;
+}
+
+/* Pair (i,k) at the mapped point u: rho = phi_i phi_k and its Gaussian-fit
+ * counterpart.  src/compute_densities.F90:16-49. */
+static void pair_density(int i, int k, const double *u, double *rho, double *rhog)
+{
+    int oi = i - 1, ok = k - 1;
+    double ri, rk;
+    double pi_ = poly_at(oi, u[0] - g_center[3 * oi], u[1] - g_center[3 * oi + 1],
+                         u[2] - g_center[3 * oi + 2], &ri);
+    double pk_ = poly_at(ok, u[0] - g_center[3 * ok], u[1] - g_center[3 * ok + 1],
+                         u[2] - g_center[3 * ok + 2], &rk);
+    double pik = pi_ * pk_;
+    *rho = (pik * zvo_u_orb(i, ri)) * zvo_u_orb(k, rk);
+    *rhog = (pik * zvo_u_gauss(i, ri)) * zvo_u_gauss(k, rk);
+}
+
+/* Pair transform, densities and Jacobians for ns samples of one pair (i>=k).
+ *   G_center: src/two_e_slater.F90:339-347
+ *   u_ik(r) = f r |r| + G: src/two_e_slater.F90:406-411
+ *   densities: src/compute_densities.F90:1-93
+ *   Jacobian 0.25 |(a/(zi+zk)) |r||^3: src/compute_jacobian_and_pi0.F90:19-31 */
+void zvo_pair_samples(int i, int k, int ns, const double *r1, const double *r2, double *ut1,
+                      double *ut2, double *rho1, double *rho2, double *rhog1, double *rhog2,
+                      double *jac1, double *jac2)
+{
+    int oi = i - 1, ok = k - 1;
+    double zi = g_zeta[oi], zk = g_zeta[ok];
+    double G[3];
+    for (int d = 0; d < 3; d++)
+        G[d] = (zi * g_center[3 * oi + d] + zk * g_center[3 * ok + d]) / (zi + zk);
+    double factor = 0.5 * g_azv / (zi + zk);
+    double factor_ik = g_azv / (zi + zk);
+    for (int m = 0; m < ns; m++) {
+        const double *a = r1 + 3 * m, *b = r2 + 3 * m;
+        double m1 = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+        double m2 = (b[0] * b[0] + b[1] * b[1]) + b[2] * b[2];
+        double r1m = sqrt(m1), r2m = sqrt(m2);
+        double u1[3], u2[3];
+        for (int d = 0; d < 3; d++) {
+            u1[d] = (factor * a[d]) * r1m + G[d];
+            u2[d] = (factor * b[d]) * r2m + G[d];
+        }
+        if (ut1) { ut1[3 * m] = u1[0]; ut1[3 * m + 1] = u1[1]; ut1[3 * m + 2] = u1[2]; }
+        if (ut2) { ut2[3 * m] = u2[0]; ut2[3 * m + 1] = u2[1]; ut2[3 * m + 2] = u2[2]; }
+        pair_density(i, k, u1, &rho1[m], &rhog1[m]);
+        pair_density(i, k, u2, &rho2[m], &rhog2[m]);
+        double f1 = factor_ik * r1m;
+        double f2 = factor_ik * r2m;
+        jac1[m] = 0.25 * fabs((f1 * f1) * f1);
+        jac2[m] = 0.25 * fabs((f2 * f2) * f2);
+    }
+}
+
+/* pi_0 = (2 pi)^-3 exp(-(|r1|^2+|r2|^2)/2): src/compute_jacobian_and_pi0.F90:8-17 */
+void zvo_pi0(int ns, const double *r1, const double *r2, double *pi0)
+{
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    const double factor = 1.0 / ((two_pi * two_pi) * two_pi);
+    for (int m = 0; m < ns; m++) {
+        const double *a = r1 + 3 * m, *b = r2 + 3 * m;
+        double m1 = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+        double m2 = (b[0] * b[0] + b[1] * b[1]) + b[2] * b[2];
+        pi0[m] = zv_exp(-0.5 * (m1 + m2)) * factor;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* quartet indexing: src/build_mapping_ijkl.F90:9-44                   */
+/* ------------------------------------------------------------------ */
+int64_t zvo_nint(int n)
+{
+    int64_t np = ((int64_t)n * (n + 1)) / 2;
+    return (np * (np + 1)) / 2;
+}
+
+int64_t zvo_build_mapping(int n, int *is, int *js, int *ks, int *ls)
+{
+    int64_t kcp = 0;
+    for (int l = 1; l <= n; l++)
+        for (int k = 1; k <= n; k++)
+            for (int j = l; j <= n; j++)
+                for (int i = k; i <= n; i++) {
+                    if (i == j && l > k) continue;
+                    if (i < j) continue;
+                    is[kcp] = i; js[kcp] = j; ks[kcp] = k; ls[kcp] = l;
+                    kcp++;
+                }
+    return kcp;
+}
+
+/* src/compare_nuclei.F90:1-139: the number of distinct nuclei among the
+ * four.  (The reference's "1xy1" branch tests nj==nk where nj/=nk is meant;
+ * that pattern cannot occur for canonical quartets because the AOs of one
+ * atom are contiguous, so counting distinct nuclei is equivalent.) */
+int zvo_compare_nuclei(int ni, int nj, int nk, int nl)
+{
+    int v[4] = {ni, nj, nk, nl};
+    int cnt = 0;
+    for (int a = 0; a < 4; a++) {
+        int seen = 0;
+        for (int b = 0; b < a; b++) if (v[b] == v[a]) seen = 1;
+        if (!seen) cnt++;
+    }
+    return cnt;
+}
+
+/* ------------------------------------------------------------------ */
+/* ZVMC estimator, fixed-sample mode                                   */
+/* src/two_e_slater.F90:386-496                                        */
+/* ------------------------------------------------------------------ */
+#define NW 40
+static inline int pidx(int i, int k) { return (i * (i - 1)) / 2 + (k - 1); } /* i>=k, 1-based */
+
+int zvo_mc_fixed(int nbl, int npts, const double *r1, const double *r2, const double *pi0,
+                 int64_t nq, const int *is, const int *js, const int *ks, const int *ls,
+                 double *mean, double *err, double *es_block, double *eg_block)
+{
+    int n = g_n;
+    if (npts % NW != 0 || nbl < 1) return 1;
+    int np = n * (n + 1) / 2;
+    /* per pair, per walker: ut(3), rho, rhoG, jac for both electrons */
+    double *ut1 = malloc(sizeof(double) * np * NW * 3), *ut2 = malloc(sizeof(double) * np * NW * 3);
+    double *rh1 = malloc(sizeof(double) * np * NW), *rh2 = malloc(sizeof(double) * np * NW);
+    double *rg1 = malloc(sizeof(double) * np * NW), *rg2 = malloc(sizeof(double) * np * NW);
+    double *jc1 = malloc(sizeof(double) * np * NW), *jc2 = malloc(sizeof(double) * np * NW);
+    double *eS = malloc(sizeof(double) * nq), *eG = malloc(sizeof(double) * nq);
+    double *s1 = calloc(nq, sizeof(double)), *s2 = calloc(nq, sizeof(double));
+    char *used = calloc(np, 1);
+    for (int64_t q = 0; q < nq; q++) {
+        used[pidx(is[q], ks[q])] = 1;
+        used[pidx(js[q], ls[q])] = 1;
+    }
+    for (int b = 0; b < nbl; b++) {
+        for (int64_t q = 0; q < nq; q++) { eS[q] = 0.0; eG[q] = 0.0; }
+        for (int kk = 0; kk < npts / NW; kk++) {
+            size_t m0 = (size_t)b * npts + (size_t)kk * NW;
+            const double *a = r1 + 3 * m0, *c = r2 + 3 * m0, *p0 = pi0 + m0;
+#pragma omp parallel for schedule(dynamic)
+            for (int i = 1; i <= n; i++)
+                for (int k = 1; k <= i; k++) {
+                    int p = pidx(i, k);
+                    if (!used[p]) continue;
+                    zvo_pair_samples(i, k, NW, a, c, ut1 + (size_t)p * NW * 3,
+                                     ut2 + (size_t)p * NW * 3, rh1 + (size_t)p * NW,
+                                     rh2 + (size_t)p * NW, rg1 + (size_t)p * NW,
+                                     rg2 + (size_t)p * NW, jc1 + (size_t)p * NW,
+                                     jc2 + (size_t)p * NW);
+                }
+#pragma omp parallel for schedule(static)
+            for (int64_t q = 0; q < nq; q++) {
+                int P = pidx(is[q], ks[q]), Q = pidx(js[q], ls[q]);
+                const double *u1 = ut1 + (size_t)P * NW * 3, *u2 = ut2 + (size_t)Q * NW * 3;
+                double sS = 0.0, sG = 0.0;
+                for (int kw = 0; kw < NW; kw++) {
+                    double dx = u1[3 * kw] - u2[3 * kw];
+                    double dy = u1[3 * kw + 1] - u2[3 * kw + 1];
+                    double dz = u1[3 * kw + 2] - u2[3 * kw + 2];
+                    double r12 = (dx * dx + dy * dy) + dz * dz;
+                    double jac = jc1[(size_t)P * NW + kw] * jc2[(size_t)Q * NW + kw];
+                    double fac = (double)((float)jac / (float)p0[kw]);
+                    double w = (fac * rh1[(size_t)P * NW + kw]) * rh2[(size_t)Q * NW + kw];
+                    double wg = (fac * rg1[(size_t)P * NW + kw]) * rg2[(size_t)Q * NW + kw];
+                    double rinv = (double)(1.0f / sqrtf((float)r12));
+                    sS = sS + w * rinv;
+                    sG = sG + wg * rinv;
+                }
+                eS[q] = eS[q] + sS;
+                eG[q] = eG[q] + sG;
+            }
+        }
+        double factor = 1.0 / (double)npts;
+        for (int64_t q = 0; q < nq; q++) {
+            double es = eS[q] * factor, eg = eG[q] * factor;
+            if (es_block) es_block[(size_t)b * nq + q] = es;
+            if (eg_block) eg_block[(size_t)b * nq + q] = eg;
+            double x = es - eg;
+            s1[q] = s1[q] + x;
+            s2[q] = s2[q] + x * x;
+        }
+    }
+    for (int64_t q = 0; q < nq; q++) {
+        double m1 = s1[q] / nbl, m2 = s2[q] / nbl;
+        mean[q] = m1;
+        err[q] = sqrt(fabs(m2 - m1 * m1)) / sqrt((double)nbl);
+    }
+    free(ut1); free(ut2); free(rh1); free(rh2); free(rg1); free(rg2); free(jc1); free(jc2);
+    free(eS); free(eG); free(s1); free(s2); free(used);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* analytic Gaussian-fit ERIs                                          */
+/* ------------------------------------------------------------------ */
+#define MAXP 96
+static const double PI_5_2 = 34.9868366552; /* truncated constant of src/bielec_integral.F90:29 */
+static const double SQPI = 1.77245385090551602729816748334;
+static const double THRESH = 1.0e-15;
+
+/* src/multiply_poly.F90:1-40 : D += B*C, nd = trimmed degree scanned from nb+nc */
+static void multiply_poly(const double *b, int nb, const double *c, int nc, double *d, int *nd)
+{
+    if ((nc | nb) < 0) return;
+    int ndtmp = nb + nc;
+    for (int ic = 0; ic <= nc; ic++) d[ic] = d[ic] + c[ic] * b[0];
+    for (int ib = 1; ib <= nb; ib++) {
+        d[ib] = d[ib] + c[0] * b[ib];
+        for (int ic = 1; ic <= nc; ic++) d[ib + ic] = d[ib + ic] + c[ic] * b[ib];
+    }
+    int t;
+    for (t = ndtmp; t >= 0; t--)
+        if (d[t] != 0.0) break;
+    *nd = t;
+}
+
+/* src/add_poly_multiply.F90:1-28 : D += cst*B */
+static void add_poly_multiply(const double *b, int nb, double cst, double *d, int *nd)
+{
+    *nd = (*nd > nb) ? *nd : nb;
+    if (*nd != -1) {
+        for (int ib = 0; ib <= nb; ib++) d[ib] = d[ib] + cst * b[ib];
+        while (d[*nd] == 0.0) {
+            *nd = *nd - 1;
+            if (*nd < 0) break;
+        }
+    }
+}
+
+typedef struct {
+    double B10[3], B01[3], B00[3], C00[3], D00[3];
+} rec_t;
+
+static void i_x2(int c, const rec_t *R, double *d, int *nd, int dim);
+static void i_x1_a1(int c, const rec_t *R, double *d, int *nd, int n_pt_in);
+static void i_x1_a2(int c, const rec_t *R, double *d, int *nd, int n_pt_in);
+static void i_x1_rec(int a, int c, const rec_t *R, double *d, int *nd, int n_pt_in);
+
+/* src/I_x2_pol_mult.F90:1-85 : I(0,c) */
+static void i_x2(int c, const rec_t *R, double *d, int *nd, int dim)
+{
+    double X[MAXP], Y[MAXP];
+    int nx, ny;
+    if (c == 0) { *nd = 0; d[0] = 1.0; return; }
+    if (c < 0) { *nd = -1; return; }
+    if (c == 1) { *nd = 2; d[0] = R->D00[0]; d[1] = R->D00[1]; d[2] = R->D00[2]; return; }
+    if (c == 2) {
+        *nd = 2;
+        d[0] = R->B01[0]; d[1] = R->B01[1]; d[2] = R->B01[2];
+        ny = 2;
+        Y[0] = R->D00[0]; Y[1] = R->D00[1]; Y[2] = R->D00[2];
+        multiply_poly(Y, ny, R->D00, 2, d, nd);
+        return;
+    }
+    for (int ix = 0; ix <= c + c; ix++) X[ix] = 0.0;
+    nx = 0;
+    i_x2(c - 2, R, X, &nx, dim);
+    for (int ix = 0; ix <= nx; ix++) X[ix] = X[ix] * (double)(c - 1);
+    multiply_poly(X, nx, R->B01, 2, d, nd);
+    ny = 0;
+    for (int ix = 0; ix <= c + c; ix++) Y[ix] = 0.0;
+    i_x2(c - 1, R, Y, &ny, dim);
+    multiply_poly(Y, ny, R->D00, 2, d, nd);
+}
+
+/* src/I_x1_pol_mult_a1.F90:1-59 : I(1,c) */
+static void i_x1_a1(int c, const rec_t *R, double *d, int *nd, int n_pt_in)
+{
+    double X[MAXP], Y[MAXP];
+    int nx, ny;
+    if (c < 0 || *nd < 0) { *nd = -1; return; }
+    nx = *nd;
+    for (int ix = 0; ix <= n_pt_in; ix++) X[ix] = 0.0;
+    i_x2(c - 1, R, X, &nx, n_pt_in);
+    if (c > 1)
+        for (int ix = 0; ix <= nx; ix++) X[ix] = X[ix] * (double)c;
+    multiply_poly(X, nx, R->B00, 2, d, nd);
+    ny = 0;
+    for (int ix = 0; ix <= n_pt_in; ix++) Y[ix] = 0.0;
+    i_x2(c, R, Y, &ny, n_pt_in);
+    multiply_poly(Y, ny, R->C00, 2, d, nd);
+}
+
+/* src/I_x1_pol_mult_a2.F90:1-61 : I(2,c) */
+static void i_x1_a2(int c, const rec_t *R, double *d, int *nd, int n_pt_in)
+{
+    double X[MAXP], Y[MAXP];
+    int nx, ny;
+    for (int ix = 0; ix <= n_pt_in; ix++) X[ix] = 0.0;
+    nx = 0;
+    i_x2(c, R, X, &nx, n_pt_in);
+    multiply_poly(X, nx, R->B10, 2, d, nd);
+    nx = *nd;
+    for (int ix = 0; ix <= n_pt_in; ix++) X[ix] = 0.0;
+    i_x1_a1(c - 1, R, X, &nx, n_pt_in);
+    if (c > 1)
+        for (int ix = 0; ix <= nx; ix++) X[ix] = X[ix] * (double)c;
+    multiply_poly(X, nx, R->B00, 2, d, nd);
+    ny = 0;
+    for (int ix = 0; ix <= n_pt_in; ix++) Y[ix] = 0.0;
+    i_x1_a1(c, R, Y, &ny, n_pt_in);
+    multiply_poly(Y, ny, R->C00, 2, d, nd);
+}
+
+/* src/I_x1_pol_mult_recurs.F90:1-80 : I(a,c), a>=3 */
+static void i_x1_rec(int a, int c, const rec_t *R, double *d, int *nd, int n_pt_in)
+{
+    double X[MAXP], Y[MAXP];
+    int nx, ny;
+    for (int ix = 0; ix <= n_pt_in; ix++) X[ix] = 0.0;
+    nx = 0;
+    if (a == 3) i_x1_a1(c, R, X, &nx, n_pt_in);
+    else if (a == 4) i_x1_a2(c, R, X, &nx, n_pt_in);
+    else i_x1_rec(a - 2, c, R, X, &nx, n_pt_in);
+    for (int ix = 0; ix <= nx; ix++) X[ix] = X[ix] * (double)(a - 1);
+    multiply_poly(X, nx, R->B10, 2, d, nd);
+    nx = *nd;
+    for (int ix = 0; ix <= n_pt_in; ix++) X[ix] = 0.0;
+    if (c > 0) {
+        if (a == 3) i_x1_a2(c - 1, R, X, &nx, n_pt_in);
+        else i_x1_rec(a - 1, c - 1, R, X, &nx, n_pt_in);
+        if (c > 1)
+            for (int ix = 0; ix <= nx; ix++) X[ix] = X[ix] * c;
+        multiply_poly(X, nx, R->B00, 2, d, nd);
+    }
+    ny = 0;
+    for (int ix = 0; ix <= n_pt_in; ix++) Y[ix] = 0.0;
+    if (a == 3) i_x1_a2(c, R, Y, &ny, n_pt_in);
+    else i_x1_rec(a - 1, c, R, Y, &ny, n_pt_in);
+    multiply_poly(Y, ny, R->C00, 2, d, nd);
+}
+
+/* src/I_x1_pol_mult.F90:1-47 */
+static void i_x1(int a, int c, const rec_t *R, double *d, int *nd, int n_pt_in)
+{
+    if (c >= 0 && *nd >= 0) {
+        if (a == 1) i_x1_a1(c, R, d, nd, n_pt_in);
+        else if (a == 2) i_x1_a2(c, R, d, nd, n_pt_in);
+        else if (a > 2) i_x1_rec(a, c, R, d, nd, n_pt_in);
+        else {
+            if (c == 0) { *nd = 0; d[0] = 1.0; return; }
+            i_x2(c, R, d, nd, n_pt_in);
+        }
+    } else {
+        *nd = -1;
+    }
+}
+
+/* src/give_polynom_mult_center_x.F90:1-74 */
+static void give_polynom_mult_center_x(double Pc, double Qc, int a_x, int d_x, double p, double q,
+                                       int n_pt_in, double pq_inv, double pq_inv_2, double p10_1,
+                                       double p01_1, double p10_2, double p01_2, double *d,
+                                       int *n_pt_out)
+{
+    rec_t R;
+    R.B10[0] = p10_1; R.B10[1] = 0.0; R.B10[2] = -p10_2;
+    R.B01[0] = p01_1; R.B01[1] = 0.0; R.B01[2] = -p01_2;
+    R.B00[0] = 0.0; R.B00[1] = 0.0; R.B00[2] = pq_inv;
+    for (int i = 0; i <= n_pt_in; i++) d[i] = 0.0;
+    int n_pt1 = n_pt_in;
+    R.C00[0] = 0.0; R.C00[1] = 0.0; R.C00[2] = -q * (Pc - Qc) * pq_inv_2;
+    R.D00[0] = 0.0; R.D00[1] = 0.0; R.D00[2] = -p * (Qc - Pc) * pq_inv_2;
+    i_x1(a_x, d_x, &R, d, &n_pt1, n_pt_in);
+    *n_pt_out = n_pt1;
+    if (n_pt1 < 0) {
+        *n_pt_out = -1;
+        for (int i = 0; i <= n_pt_in; i++) d[i] = 0.0;
+    }
+}
+
+/* src/gaussian_product.F90:1-32 */
+static void gaussian_product(double a, const double *xa, double b, const double *xb, double *k,
+                             double *p, double *xp)
+{
+    *p = a + b;
+    double p_inv = 1.0 / (a + b);
+    double ab = a * b;
+    double xab[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+    ab = ab * p_inv;
+    *k = ab * ((xab[0] * xab[0] + xab[1] * xab[1]) + xab[2] * xab[2]);
+    if (*k > 40.0) { *k = 0.0; return; }
+    *k = exp(-*k);
+    xp[0] = (a * xa[0] + b * xb[0]) * p_inv;
+    xp[1] = (a * xa[1] + b * xb[1]) * p_inv;
+    xp[2] = (a * xa[2] + b * xb[2]) * p_inv;
+}
+
+static double binom_d(int n, int k)
+{
+    double r = 1.0;
+    for (int t = 1; t <= k; t++) r = r * (double)(n - k + t) / (double)t;
+    return r;
+}
+
+/* src/give_explicit_poly_and_gaussian.F90:86-220 (recentered_poly):
+ * (x-A)^a = sum_i binom(a,i) (P-A)^(a-i) (x-P)^i.  Powers built as the
+ * reference does for a<=8 (pows(4) = pows(2)^2), sequentially otherwise. */
+static void recentered_poly(double *P_new, double xA, double xP, int a)
+{
+    double pw[32];
+    if (a < 0) return;
+    pw[0] = 1.0;
+    pw[1] = xP - xA;
+    if (a == 0) { P_new[0] = pw[0]; return; }
+    if (a == 1) { P_new[0] = pw[1]; P_new[1] = pw[0]; return; }
+    if (a == 4) {
+        pw[2] = pw[1] * pw[1];
+        pw[3] = pw[2] * pw[1];
+        pw[4] = pw[2] * pw[2];
+    } else {
+        for (int i = 2; i <= a; i++) pw[i] = pw[i - 1] * pw[1];
+    }
+    P_new[0] = pw[a];
+    for (int i = 1; i < a; i++) P_new[i] = binom_d(a, i) * pw[a - i];
+    P_new[a] = pw[0];
+}
+
+/* src/give_explicit_poly_and_gaussian.F90:1-84 */
+static void give_explicit_poly_and_gaussian(double P_new[3][MAXP], double *P_center, double *p,
+                                            double *fact_k, int *iorder, double alpha, double beta,
+                                            const int *a, const int *b, const double *A,
+                                            const double *B)
+{
+    iorder[0] = iorder[1] = iorder[2] = 0;
+    P_new[0][0] = P_new[1][0] = P_new[2][0] = 0.0;
+    gaussian_product(alpha, A, beta, B, fact_k, p, P_center);
+    if (*fact_k < THRESH) { *fact_k = 0.0; return; }
+    for (int ax = 0; ax < 3; ax++) {
+        double Pa[MAXP], Pb[MAXP];
+        recentered_poly(Pa, A[ax], P_center[ax], a[ax]);
+        recentered_poly(Pb, B[ax], P_center[ax], b[ax]);
+        iorder[ax] = a[ax] + b[ax];
+        for (int i = 0; i <= iorder[ax]; i++) P_new[ax][i] = 0.0;
+        int n_new = 0;
+        multiply_poly(Pa, a[ax], Pb, b[ax], P_new[ax], &n_new);
+    }
+}
+
+/* src/rint1.F90:1-59 (series for rho < 1).  The reference indexes a
+ * 100-entry table of 1/k; for 2(k+n)+1 > 100 it reads past the table, here
+ * 1/(2(k+n)+1) is used instead (not reached for l <= 4 per orbital). */
+static double fact_inv(int k)
+{
+    double f = 1.0;
+    for (int t = 2; t <= k; t++) f = f * (double)t;
+    return 1.0 / f;
+}
+
+static double rint1(int n, double rho)
+{
+    const double eps = 1.0e-15;
+    double r, rho_tmp = 1.0, diff = 0.0;
+    if (n < 50) {
+        r = 1.0 / (double)(n + n + 1);
+        for (int k = 1; k <= 20; k++) {
+            rho_tmp = -rho_tmp * rho;
+            int idx = 2 * (k + n) + 1;
+            double inv = 1.0 / (double)idx;
+            diff = (rho_tmp * fact_inv(k)) * inv;
+            r = r + diff;
+            if (fabs(diff) > eps) continue;
+            return r;
+        }
+    } else {
+        r = 1.0 / (double)(n + n + 1);
+        for (int k = 1; k <= 20; k++) {
+            rho_tmp = -rho_tmp * rho;
+            diff = (rho_tmp * fact_inv(k)) / (double)(2 * (k + n) + 1);
+            r = r + diff;
+            if (fabs(diff) > eps) continue;
+            return r;
+        }
+    }
+    fprintf(stderr, "zvo rint1: series did not converge n=%d rho=%g\n", n, rho);
+    return r;
+}
+
+static double fact_d(int n)
+{
+    double f = 1.0;
+    for (int t = 2; t <= n; t++) f = f * (double)t;
+    return f;
+}
+
+/* src/hermite.F90 */
+static double hermite(int n, double x)
+{
+    double h0 = 1.0, h1, h2 = 0.0;
+    if (n == 0) return h0;
+    h1 = x + x;
+    if (n == 1) return h1;
+    for (int k = 1; k <= n - 1; k++) {
+        h2 = (x + x) * h1 - (double)(k + k) * h0;
+        h0 = h1;
+        h1 = h2;
+    }
+    return h2;
+}
+
+/* src/rint_large_n.F90:1-29 */
+static double rint_large_n(int n, double rho)
+{
+    double u = sqrt(rho), accu = 0.0, eps = 1.0;
+    int k = 0;
+    while (eps > 1.0e-15) {
+        double t1 = 1.0;
+        for (int l = 0; l <= k; l++) t1 = t1 * (n + n + l + 1.0);
+        double t2 = 0.0;
+        for (int l = 0; l <= k; l++) t2 = t2 + ((l & 1) ? -1.0 : 1.0) / (fact_d(l + 1) * fact_d(k - l));
+        double alpha_k = t2 * fact_d(k + 1) * fact_d(k) * ((k & 1) ? -1.0 : 1.0);
+        alpha_k = alpha_k / t1;
+        double rajout = ((k & 1) ? -1.0 : 1.0) * pow(u, k) * hermite(k, u) * alpha_k / fact_d(k);
+        accu = accu + rajout;
+        eps = fabs(rajout) / accu;
+        k++;
+        if (k > 400) break;
+    }
+    return exp(-rho) * accu;
+}
+
+/* src/rint_sum.F90:1-67 : sum_i d1(2i) F_i(rho) */
+double zvo_rint_sum(int n_pt_out, double rho, const double *d1)
+{
+    double s;
+    if (rho < 1.0) {
+        if (rho == 0.0) s = d1[0];
+        else {
+            double u_inv = 1.0 / sqrt(rho);
+            double u = rho * u_inv;
+            s = (((0.5 * u_inv) * SQPI) * erf(u)) * d1[0];
+        }
+        for (int i = 2; i <= n_pt_out; i += 2) s = s + d1[i] * rint1(i >> 1, rho);
+    } else {
+        double v = (rho > 80.0) ? 0.0 : exp(-rho);
+        double u_inv = 1.0 / sqrt(rho);
+        double u = rho * u_inv;
+        double two_rho_inv = (0.5 * u_inv) * u_inv;
+        double val0 = ((0.5 * u_inv) * SQPI) * erf(u);
+        s = val0 * d1[0];
+        double rint_tmp = (val0 - v) * two_rho_inv;
+        double di = 3.0;
+        int lim = n_pt_out < 40 ? n_pt_out : 40;
+        for (int i = 2; i <= lim; i += 2) {
+            s = s + d1[i] * rint_tmp;
+            rint_tmp = (rint_tmp * di - v) * two_rho_inv;
+            di = di + 2.0;
+        }
+        for (int i = 42; i <= n_pt_out; i += 2) s = s + d1[i] * rint_large_n(i >> 1, rho);
+    }
+    return s;
+}
+
+/* src/rint.F90:1-62 : F_n(rho) = int_0^1 t^2n exp(-rho t^2) dt */
+double zvo_rint(int n, double rho)
+{
+    if (n == 0) {
+        if (rho == 0.0) return 1.0;
+        double u_inv = 1.0 / sqrt(rho);
+        double u = rho * u_inv;
+        return ((0.5 * u_inv) * SQPI) * erf(u);
+    }
+    if (rho < 1.0) return rint1(n, rho);
+    if (n <= 20) {
+        double u_inv = 1.0 / sqrt(rho);
+        double v = (rho > 80.0) ? 0.0 : exp(-rho);
+        double u = rho * u_inv;
+        double two_rho_inv = (0.5 * u_inv) * u_inv;
+        double val0 = ((0.5 * u_inv) * SQPI) * erf(u);
+        double r = (val0 - v) * two_rho_inv;
+        for (int k = 2; k <= n; k++) r = (r * (double)(k + k - 1) - v) * two_rho_inv;
+        return r;
+    }
+    return rint_large_n(n, rho);
+}
+
+/* src/general_primitive_integral.F90:9-194 */
+static double general_primitive_integral(double P_new[3][MAXP], const double *P_center,
+                                         double fact_p, double p, double p_inv,
+                                         const int *iorder_p, double Q_new[3][MAXP],
+                                         const double *Q_center, double fact_q, double q,
+                                         double q_inv, const int *iorder_q)
+{
+    double pq = (p_inv * 0.5) * q_inv;
+    double pq_inv = 0.5 / (p + q);
+    double p10_1 = q * pq;
+    double p01_1 = p * pq;
+    double pq_inv_2 = pq_inv + pq_inv;
+    double p10_2 = (pq_inv_2 * p10_1) * q;
+    double p01_2 = (pq_inv_2 * p01_1) * p;
+    double I[3][MAXP], dd[MAXP];
+    int nI[3];
+    for (int ax = 0; ax < 3; ax++) {
+        int iorder = iorder_p[ax] + iorder_q[ax] + iorder_p[ax] + iorder_q[ax];
+        for (int t = 0; t <= iorder; t++) I[ax][t] = 0.0;
+        nI[ax] = 0;
+        for (int ix = 0; ix <= iorder_p[ax]; ix++) {
+            double a = P_new[ax][ix];
+            if (ax == 0) { if (fabs(a) < THRESH) continue; }
+            else { if (!(fabs(a) > THRESH)) continue; }
+            for (int jx = 0; jx <= iorder_q[ax]; jx++) {
+                double dcoef = a * Q_new[ax][jx];
+                if (fabs(dcoef) < THRESH) continue;
+                int nx;
+                give_polynom_mult_center_x(P_center[ax], Q_center[ax], ix, jx, p, q, iorder,
+                                           pq_inv, pq_inv_2, p10_1, p01_1, p10_2, p01_2, dd, &nx);
+                add_poly_multiply(dd, nx, dcoef, I[ax], &nI[ax]);
+            }
+        }
+        if (nI[ax] == -1) return 0.0;
+    }
+    double rho = (p * q) * pq_inv_2;
+    double dx = P_center[0] - Q_center[0], dy = P_center[1] - Q_center[1],
+           dz = P_center[2] - Q_center[2];
+    double dist = (dx * dx + dy * dy) + dz * dz;
+    double cst = dist * rho;
+    double d_poly[MAXP], d1[MAXP];
+    int n_pt_tmp = nI[0] + nI[1];
+    for (int i = 0; i <= n_pt_tmp; i++) d_poly[i] = 0.0;
+    multiply_poly(I[0], nI[0], I[1], nI[1], d_poly, &n_pt_tmp);
+    if (n_pt_tmp == -1) return 0.0;
+    int n_pt_out = n_pt_tmp + nI[2];
+    for (int i = 0; i <= n_pt_out; i++) d1[i] = 0.0;
+    multiply_poly(d_poly, n_pt_tmp, I[2], nI[2], d1, &n_pt_out);
+    double accu = 0.0 + zvo_rint_sum(n_pt_out, cst, d1);
+    return ((((fact_p * fact_q) * accu) * PI_5_2) * p_inv) * q_inv / sqrt(p + q);
+}
+
+/* src/bielec_integral.F90:1-55 */
+double zvo_bielec_integral(double alpha, double beta, double delta, double gama,
+                           const double *A, const double *B, const double *C, const double *D,
+                           const int *pA, const int *pB, const int *pC, const int *pD)
+{
+    double P_new[3][MAXP], Q_new[3][MAXP], Pc[3] = {0, 0, 0}, Qc[3] = {0, 0, 0};
+    double pp, qq, fp, fq;
+    int ip[3], iq[3];
+    give_explicit_poly_and_gaussian(P_new, Pc, &pp, &fp, ip, alpha, beta, pA, pB, A, B);
+    double p_inv = 1.0 / pp;
+    give_explicit_poly_and_gaussian(Q_new, Qc, &qq, &fq, iq, delta, gama, pC, pD, C, D);
+    double q_inv = 1.0 / qq;
+    if (fp == 0.0 || fq == 0.0) return 0.0; /* all polynomials vanish: reference returns 0 */
+    return general_primitive_integral(P_new, Pc, fp, pp, p_inv, ip, Q_new, Qc, fq, qq, q_inv, iq);
+}
+
+/* src/gauss_ijkl.F90:51-154 : contracted (ik|jl) over the Gaussian fits;
+ * i_type 2 expands r^2 = x^2+y^2+z^2 into three Cartesian terms. */
+double zvo_gauss_ijkl(int i, int k, int j, int l)
+{
+    int orb[4] = {i - 1, k - 1, j - 1, l - 1};
+    int nc[4], n_c[4][4][3];
+    const double *dcoef[4][4];
+    for (int kk = 0; kk < 4; kk++) {
+        int o = orb[kk];
+        const int *np_ = g_npower + 3 * o;
+        const double *c1 = g_c1 + (size_t)g_ldg * o, *c2 = g_c2 + (size_t)g_ldg * o;
+        if (g_itype[o] == 1) {
+            nc[kk] = 1;
+            dcoef[kk][0] = c1;
+            for (int ll = 0; ll < 3; ll++) n_c[kk][0][ll] = np_[ll];
+        } else if (g_itype[o] == 2) {
+            nc[kk] = 3;
+            for (int ii = 0; ii < 3; ii++) {
+                dcoef[kk][ii] = c2;
+                for (int ll = 0; ll < 3; ll++) n_c[kk][ii][ll] = np_[ll] + (ll == ii ? 2 : 0);
+            }
+        } else {
+            nc[kk] = 4;
+            dcoef[kk][0] = c1;
+            for (int ll = 0; ll < 3; ll++) n_c[kk][0][ll] = np_[ll];
+            for (int ii = 1; ii < 4; ii++) {
+                dcoef[kk][ii] = c2;
+                for (int ll = 0; ll < 3; ll++) n_c[kk][ii][ll] = np_[ll] + (ll == ii - 1 ? 2 : 0);
+            }
+        }
+    }
+    const double *A = g_center + 3 * orb[0], *B = g_center + 3 * orb[1];
+    const double *C = g_center + 3 * orb[2], *D = g_center + 3 * orb[3];
+    const double *gA = g_gg + (size_t)g_ldg * orb[0], *gB = g_gg + (size_t)g_ldg * orb[1];
+    const double *gC = g_gg + (size_t)g_ldg * orb[2], *gD = g_gg + (size_t)g_ldg * orb[3];
+    double rint = 0.0;
+    for (int i1 = 0; i1 < g_ngauss[orb[0]]; i1++)
+        for (int i2 = 0; i2 < g_ngauss[orb[1]]; i2++)
+            for (int i3 = 0; i3 < g_ngauss[orb[2]]; i3++)
+                for (int i4 = 0; i4 < g_ngauss[orb[3]]; i4++)
+                    for (int ii1 = 0; ii1 < nc[0]; ii1++)
+                        for (int ii2 = 0; ii2 < nc[1]; ii2++)
+                            for (int ii3 = 0; ii3 < nc[2]; ii3++)
+                                for (int ii4 = 0; ii4 < nc[3]; ii4++) {
+                                    double dd = ((dcoef[0][ii1][i1] * dcoef[1][ii2][i2]) *
+                                                 dcoef[2][ii3][i3]) * dcoef[3][ii4][i4];
+                                    rint = rint + dd * zvo_bielec_integral(
+                                                           gA[i1], gB[i2], gC[i3], gD[i4], A, B, C,
+                                                           D, n_c[0][ii1], n_c[1][ii2],
+                                                           n_c[2][ii3], n_c[3][ii4]);
+                                }
+    return rint;
+}
+
+/* precond(i,k) = gauss_ijkl(i,k,i,k): src/two_e_slater.F90:230-235.
+ * precond[(k-1)*n + (i-1)] (Fortran precond(n,n)). */
+void zvo_precond(double *precond)
+{
+    int n = g_n;
+#pragma omp parallel for schedule(dynamic)
+    for (int kk = 0; kk < n * n; kk++) {
+        int k = kk / n + 1, i = kk % n + 1;
+        precond[kk] = zvo_gauss_ijkl(i, k, i, k);
+    }
+}
+
+/* Analytic loop with Schwarz screening: src/two_e_slater.F90:253-279 */
+void zvo_gauss_eri(int64_t nq, const int *is, const int *js, const int *ks, const int *ls,
+                   const double *precond, double *out, int64_t *n_screened)
+{
+    int n = g_n;
+    const double seuil = 1.0e-15;
+    int64_t nscr = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nscr)
+    for (int64_t q = 0; q < nq; q++) {
+        int i = is[q], k = ks[q], j = js[q], l = ls[q];
+        if (!precond) { /* no Schwarz driver: plain gauss_ijkl for each quartet */
+            out[q] = zvo_gauss_ijkl(i, k, j, l);
+            continue;
+        }
+        double pik = precond[(size_t)(k - 1) * n + (i - 1)];
+        double pjl = precond[(size_t)(l - 1) * n + (j - 1)];
+        if (fabs(pik * pjl) < seuil * seuil) nscr++;
+        if (i == j && k == l) out[q] = pik;
+        else if (fabs(pik * pjl) >= seuil * seuil) out[q] = zvo_gauss_ijkl(i, k, j, l);
+        else out[q] = 0.0;
+    }
+    if (n_screened) *n_screened = nscr;
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon et al., SC'11 "Parallel random numbers: as easy as
+ * 1, 2, 3"): the counter-based generator the device sampler uses. */
+/* ------------------------------------------------------------------ */
+void zvo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n1 = lo1;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        uint32_t n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}

@@ -1,0 +1,205 @@
+"""Write golden vectors from the compiled reference (oracle/_ref/libref.so).
+
+Run in the container where the reference tree is mounted:
+
+    make -C oracle -f Makefile.ref REF=/root/reference
+    python tests/golden/make_golden.py
+
+The library is the reference's own Fortran sources (built by
+oracle/Makefile.ref) plus the bind(C) harness oracle/ref_harness.F90.  The
+outputs are small .npz files of inputs and expected outputs; nothing of the
+reference's source is stored.  Inputs are written as the reference's own
+deck files (xyz + basis) and read back by the reference's routines.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from slater_stoch_amd import basis as B  # noqa: E402
+from slater_stoch_amd import configs  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+
+
+def load_ref():
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "_ref" / "libref.so"))
+    lib.ref_setup.restype = ctypes.c_int
+    lib.ref_setup.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    lib.ref_gauss_ijkl.restype = ctypes.c_double
+    lib.ref_gauss_ijkl.argtypes = [ctypes.c_int] * 4
+    lib.ref_rint.restype = ctypes.c_double
+    lib.ref_rint.argtypes = [ctypes.c_int, ctypes.c_double]
+    lib.ref_dierfc.restype = ctypes.c_double
+    lib.ref_dierfc.argtypes = [ctypes.c_double]
+    lib.ref_enucl.restype = ctypes.c_double
+    lib.ref_compare_nuclei.restype = ctypes.c_int
+    lib.ref_compare_nuclei.argtypes = [ctypes.c_int] * 4
+    lib.ref_build_mapping.restype = ctypes.c_int64
+    lib.ref_get_basis.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 10
+    lib.ref_mc_fixed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 6
+    lib.ref_pair_samples.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 9
+    return lib
+
+
+def setup(lib, bas, name, ng0, **kw):
+    d = Path(tempfile.mkdtemp())
+    configs.write_deck(bas, d, name=name, ng0=ng0)
+    n = lib.ref_setup(str(d / name).encode(), str(d / f"{name}.basis").encode(), ng0,
+                      kw.get("i_add_2p", 0), kw.get("i_add_3d", 0), kw.get("g_thr", 100.0),
+                      kw.get("i_add_large_g", 0), bas.a_zv)
+    assert n == bas.n, (n, bas.n)
+    return n
+
+
+def get_basis(lib, n, ldg):
+    a = dict(center=np.zeros((n, 3)), npower=np.zeros((n, 3), np.int32),
+             n_sto=np.zeros(n, np.int32), i_type=np.zeros(n, np.int32), zeta=np.zeros(n),
+             n_gauss=np.zeros(n, np.int32), nucleus=np.zeros(n, np.int32),
+             c1=np.zeros((n, ldg)), c2=np.zeros((n, ldg)), g=np.zeros((n, ldg)))
+    lib.ref_get_basis(ldg, P(a["center"]), P(a["npower"]), P(a["n_sto"]), P(a["i_type"]),
+                      P(a["zeta"]), P(a["n_gauss"]), P(a["nucleus"]), P(a["c1"]), P(a["c2"]),
+                      P(a["g"]))
+    return a
+
+
+def mapping(lib, n):
+    nint = (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+    q = [np.zeros(nint, np.int32) for _ in range(4)]
+    got = lib.ref_build_mapping(P(q[0]), P(q[1]), P(q[2]), P(q[3]))
+    assert got == nint
+    return q
+
+
+def analytic_driver(lib, n, is_, js_, ks_, ls_):
+    """precond + Schwarz + diagonal reuse as src/two_e_slater.F90:219-279."""
+    pre = np.zeros((n, n))  # pre[k-1, i-1] == Fortran precond(i,k)
+    for k in range(1, n + 1):
+        for i in range(1, n + 1):
+            pre[k - 1, i - 1] = lib.ref_gauss_ijkl(i, k, i, k)
+    out = np.zeros(is_.size)
+    for q in range(is_.size):
+        i, j, k, l = int(is_[q]), int(js_[q]), int(ks_[q]), int(ls_[q])
+        pik, pjl = pre[k - 1, i - 1], pre[l - 1, j - 1]
+        if i == j and k == l:
+            out[q] = pik
+        elif abs(pik * pjl) >= 1e-30:
+            out[q] = lib.ref_gauss_ijkl(i, k, j, l)
+    return pre, out
+
+
+def samples(seed, m):
+    rng = np.random.default_rng(seed)
+    r1 = rng.standard_normal((m, 3))
+    r2 = rng.standard_normal((m, 3))
+    f = 1.0 / (2.0 * np.pi) ** 3
+    pi0 = np.exp(-0.5 * ((r1 ** 2).sum(1) + (r2 ** 2).sum(1))) * f
+    return r1, r2, pi0
+
+
+def mc_fixed(lib, nbl, npts, r1, r2, pi0, q):
+    nq = q[0].size
+    mean, err = np.zeros(nq), np.zeros(nq)
+    lib.ref_mc_fixed(nbl, npts, P(np.ascontiguousarray(r1)), P(np.ascontiguousarray(r2)),
+                     P(pi0), nq, P(q[0]), P(q[1]), P(q[2]), P(q[3]), P(mean), P(err))
+    return mean, err
+
+
+def main():
+    lib = load_ref()
+
+    # --- water, minimal basis, 20 Gaussians per STO --------------------
+    bas = configs.water_minimal(ng0=20)
+    n = setup(lib, bas, "h2o", 20)
+    rb = get_basis(lib, n, bas.ldg)
+    q = mapping(lib, n)
+    print("water: analytic ERIs via the reference (about a minute)", flush=True)
+    pre, eri = analytic_driver(lib, n, *q)
+    np.savez(OUT / "water_ng20_basis.npz", enucl=lib.ref_enucl(), **rb)
+    np.savez(OUT / "water_ng20_eri.npz", is_=q[0], js=q[1], ks=q[2], ls=q[3], precond=pre,
+             eri=eri)
+    # fixed-sample MC: 10 blocks x 40 samples, and 4 blocks x 200 samples
+    r1, r2, pi0 = samples(5, 400)
+    mean, err = mc_fixed(lib, 10, 40, r1, r2, pi0, q)
+    r1b, r2b, pi0b = samples(6, 800)
+    meanb, errb = mc_fixed(lib, 4, 200, r1b, r2b, pi0b, q)
+    np.savez(OUT / "water_ng20_mc.npz", r1=r1, r2=r2, pi0=pi0, mean=mean, err=err, r1b=r1b,
+             r2b=r2b, pi0b=pi0b, meanb=meanb, errb=errb)
+    # per-pair densities / Jacobians on 80 samples for a few pairs
+    pairs = [(1, 1), (2, 1), (3, 2), (5, 3), (6, 1), (7, 6), (7, 7), (4, 4)]
+    rec = {}
+    for (i, k) in pairs:
+        arrs = [np.zeros(80) for _ in range(7)]
+        lib.ref_pair_samples(i, k, 80, P(np.ascontiguousarray(r1[:80])),
+                             P(np.ascontiguousarray(r2[:80])), *[P(a) for a in arrs])
+        for name, a in zip(["rho1", "rho2", "rhog1", "rhog2", "jac1", "jac2", "pi0"], arrs):
+            rec[f"{name}_{i}_{k}"] = a
+    np.savez(OUT / "water_ng20_pairs.npz", pairs=np.array(pairs, np.int32), r1=r1[:80],
+             r2=r2[:80], **rec)
+
+    # --- all orbital types through the analytic engine, 4 Gaussians -----
+    geom = B.geometry_from_angstrom(["O", "H", "N"],
+                                    [[0.1, -0.2, 0.05], [0.3, 1.05, -0.4], [-1.1, 0.2, 0.7]])
+    basis = {"O": [("1S", 7.7), ("2S", 2.2), ("3S", 1.4), ("2P", 2.3), ("3P", 1.1), ("3D", 1.9)],
+             "H": [("1S", 1.3), ("2P", 1.6)],
+             "N": [("4F", 1.7), ("5G", 2.1)]}
+    bas = B.expand_basis(geom, basis, ng0=4)
+    n = setup(lib, bas, "mixed", 4)
+    rb = get_basis(lib, n, bas.ldg)
+    q = mapping(lib, n)
+    rng = np.random.default_rng(11)
+    pick = np.sort(rng.choice(q[0].size, 3000, replace=False))
+    print(f"mixed: {n} AOs, {q[0].size} quartets, {pick.size} picked", flush=True)
+    vals = np.array([lib.ref_gauss_ijkl(int(q[0][t]), int(q[2][t]), int(q[1][t]), int(q[3][t]))
+                     for t in pick])
+    np.savez(OUT / "mixed_ng4_eri.npz", pick=pick, eri=vals, **rb)
+
+    # --- 12-atom / 110-AO molecule: stratified subset, 20 Gaussians -----
+    bas = configs.random_molecule()
+    n = setup(lib, bas, "rand12", 20)
+    rb = get_basis(lib, n, bas.ldg)
+    rng = np.random.default_rng(12)
+    cls = []
+    chosen = []
+    type2 = bas.i_type == 2
+    while len(chosen) < 48:
+        i, k, j, l = sorted(rng.integers(1, n + 1, 2))[::-1] + sorted(rng.integers(1, n + 1, 2))[::-1]
+        if (i, k) < (j, l):
+            i, k, j, l = j, l, i, k
+        if type2[[i - 1, k - 1, j - 1, l - 1]].sum() > 1:
+            continue
+        c = B.nuclei_class(bas, np.array([i]), np.array([j]), np.array([k]), np.array([l]))[0]
+        if cls.count(c) >= 12:
+            continue
+        cls.append(c)
+        chosen.append((i, k, j, l))
+    chosen = np.array(chosen, np.int32)
+    print("rand12: 48 quartets", flush=True)
+    vals = np.array([lib.ref_gauss_ijkl(*map(int, t)) for t in chosen])
+    np.savez(OUT / "rand12_ng20_eri.npz", quartets=chosen, eri=vals, cls=np.array(cls),
+             n=n, zeta=rb["zeta"], center=rb["center"], c1=rb["c1"], c2=rb["c2"], g=rb["g"])
+
+    # --- Boys-type integrals and the inverse erfc --------------------------
+    ns = np.arange(0, 17)
+    rhos = np.array([0.0, 1e-8, 1e-3, 0.3, 0.999, 1.0, 2.5, 11.0, 37.0, 79.9, 80.1, 240.0])
+    F = np.array([[lib.ref_rint(int(nn), float(r)) for r in rhos] for nn in ns])
+    ys = np.concatenate([np.linspace(1e-6, 1.999999, 201), [1e-12, 1e-3, 1.0, 1.5, 2 - 1e-12]])
+    dv = np.array([lib.ref_dierfc(float(y)) for y in ys])
+    cn = np.array([[a, b, c, d, lib.ref_compare_nuclei(a, b, c, d)]
+                   for a in (1, 2) for b in (1, 2, 3) for c in (1, 3, 4) for d in (1, 3, 4)
+                   if not (a == d and b != c and b != a and c != a)], np.int32)
+    np.savez(OUT / "scalars.npz", n=ns, rho=rhos, rint=F, y=ys, dierfc=dv, nuclei=cn)
+    print("done")
+
+
+if __name__ == "__main__":
+    main()
