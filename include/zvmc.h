@@ -1,0 +1,131 @@
+/*
+ * zvmc.h -- C-ABI of the MI355X ZVMC two-electron-integral engine.
+ *
+ * Drop-in for the hot path of scemama/slater_stoch's two_e_slater.  The
+ * reference has no library API: its hot path is inline in the main program
+ * and in Fortran external procedures with implicit interfaces.  Each entry
+ * point below names the reference code it replaces (file:line, paths
+ * relative to the reference tree).  The Fortran host binds these with
+ * iso_c_binding (slater_stoch_amd/fortran/zvmc_mod.F90).
+ *
+ * Conventions
+ *  - Plain pointers to host memory, Fortran (column-major) layouts:
+ *      center(3,n), npower(3,n), c_gauss(ldg,n,2), g_gauss(ldg,n),
+ *      samples r(3,m) (x,y,z of sample 1, then sample 2, ...).
+ *  - Orbital indices are 1-based (as in the reference); quartet counts int64.
+ *  - Quartet arrays (ERIs, means, errors) are in the reference order of
+ *    src/build_mapping_ijkl.F90:16-43 (kcp = 1..nint).
+ *  - Every call returns 0 on success or a ZV_ERR_* code; zv_last_error()
+ *    returns the message.  The Fortran host turns nonzero into `stop`.
+ *  - One HIP stream per context; calls are synchronous for the caller.
+ *    Contexts are not thread-safe: call from one host thread.
+ */
+#ifndef ZVMC_H
+#define ZVMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZV_OK 0
+#define ZV_ERR_ARG 1     /* invalid argument or unsupported input        */
+#define ZV_ERR_HIP 2     /* HIP runtime / library error                   */
+#define ZV_ERR_STATE 3   /* call out of order (e.g. no basis set)         */
+#define ZV_ERR_NOMEM 4   /* device allocation failed                      */
+#define ZV_ERR_NODEV 5   /* no usable gfx950 device                       */
+
+/* sampling modes for zv_mc_blocks */
+#define ZV_SAMPLES_FIXED 0   /* caller-supplied (r1, r2, pi0) list        */
+#define ZV_SAMPLES_PHILOX 1  /* on-device Philox4x32-10 + inverse erfc    */
+
+/* estimator precisions for zv_mc_blocks */
+#define ZV_PREC_FP64_PARITY 0 /* reference operation order and FP32 casts,
+                                 bit-identical to the CPU oracle          */
+#define ZV_PREC_FP64 1        /* tiled FP64 estimator, split S-G form      */
+#define ZV_PREC_FP32 2        /* tiled FP32 estimator, FP64 block sums     */
+
+typedef struct zv_context zv_context;
+
+/* Context on HIP device `device` (one stream).  Replaces the module state of
+ * src/common_data.F90:1-58 that the reference keeps in globals. */
+int zv_create(int device, zv_context **ctx);
+int zv_destroy(zv_context *ctx);
+int zv_last_error(const zv_context *ctx, char *buf, int len);
+
+/* Basis as filled by src/read_basis.F90:27-98 and
+ * src/build_gaussian_expansion_of_orbitals.F90:9-471 (module common_data):
+ * center(3,n) bohr, npower(3,n), n_STO(n), i_type(n) (1 or 2), g_slater(n),
+ * n_gauss(n), c_gauss(ldg,n,2), g_gauss(ldg,n), a_ZV (src/two_e_slater.F90:153).
+ * n_gauss(i) <= 32.  Also builds the pair table (G_center of
+ * src/two_e_slater.F90:339-347) and the quartet index of
+ * src/build_mapping_ijkl.F90. */
+int zv_set_basis(zv_context *ctx, int n, const double *center, const int *npower,
+                 const int *n_sto, const int *i_type, const double *zeta, const int *n_gauss,
+                 int ldg, const double *c_gauss, const double *g_gauss, double a_zv);
+
+/* nint = np(np+1)/2 with np = n(n+1)/2 (src/build_mapping_ijkl.F90:12-13). */
+int64_t zv_nint(int n);
+
+/* is/js/ks/ls of src/build_mapping_ijkl.F90:9-44 (host computation, 1-based). */
+int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls);
+
+/* Analytic Gaussian-fit ERIs with the reference's Schwarz driver,
+ * src/two_e_slater.F90:219-284: precond(i,k) = (ik|ik) (out, n x n, may be
+ * NULL), ijkl_gaus(kcp) for all nint quartets (out), n_screened = quartets
+ * dropped by the 1e-15 Cauchy-Schwarz test (out, may be NULL).
+ * Per quartet this is gauss_ijkl (src/gauss_ijkl.F90:51-154). */
+int zv_gauss_eri(zv_context *ctx, double *ijkl_gaus, double *precond, int64_t *n_screened);
+
+/* gauss_ijkl(i,k,j,l) for an explicit list of nq quartets (1-based orbital
+ * indices; no screening).  src/gauss_ijkl.F90:51 batched. */
+int zv_gauss_ijkl_list(zv_context *ctx, int64_t nq, const int *qi, const int *qk,
+                       const int *qj, const int *ql, double *out);
+
+/* ZVMC correction blocks, src/two_e_slater.F90:386-496: nbl blocks of npts
+ * samples (npts a multiple of 40, src/two_e_slater.F90:148).  mode
+ * ZV_SAMPLES_FIXED reads r1(3,nbl*npts), r2(3,nbl*npts), pi0(nbl*npts);
+ * ZV_SAMPLES_PHILOX draws them on the device from (seed, stream)
+ * (src/draw_configuration.F90:1-29) and ignores r1/r2/pi0.  Outputs:
+ * mean(kcp) = block average of e_S - e_G, err(kcp) = the reference's
+ * standard error sqrt|<x^2>-<x>^2|/sqrt(nbl), optional block_means(kcp,nbl)
+ * (may be NULL). */
+int zv_mc_blocks(zv_context *ctx, int nbl, int64_t npts, int mode, uint64_t seed,
+                 uint64_t stream, int precision, const double *r1, const double *r2,
+                 const double *pi0, double *mean, double *err, double *block_means);
+
+/* Samples drawn by the device sampler for global sample indices
+ * m0 .. m0+m-1 (ZV_SAMPLES_PHILOX), copied to the host: r1(3,m), r2(3,m),
+ * pi0(m).  src/draw_configuration.F90:1-29 + src/compute_jacobian_and_pi0.F90:8-17. */
+int zv_draw_samples(zv_context *ctx, uint64_t seed, uint64_t stream, int64_t m0, int64_t m,
+                    double *r1, double *r2, double *pi0);
+
+/* Pair transform, densities and Jacobians for pair (i,k), i >= k, at ns
+ * caller samples r(3,ns): u_ik(r1) (3,ns), u_ik(r2), rho, rho_G at both
+ * points and both Jacobians.  src/two_e_slater.F90:401-419,
+ * src/compute_densities.F90:1-93, src/compute_jacobian_and_pi0.F90:1-33. */
+int zv_pair_samples(zv_context *ctx, int i, int k, int ns, const double *r1, const double *r2,
+                    double *ut1, double *ut2, double *rho1, double *rho2, double *rhog1,
+                    double *rhog2, double *jac1, double *jac2);
+
+/* PSD cleanup of the ERI list, src/svd_clean.F90:1-426 (rank, q power
+ * iterations, Gaussian test matrices from `seed`): eri(kcp) in/out.
+ * Removes negative eigencomponents of W((i,j),(k,l)) = <ij|kl>. */
+int zv_svd_clean(zv_context *ctx, int rank, int q, uint64_t seed, double *eri,
+                 int *n_removed);
+
+/* Negative-eigenvalue removal through a dense symmetric eigensolve of the
+ * same matrix, src/davidson_clean.F90:30-120 (dense branch). */
+int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
+
+/* Milliseconds spent in the last call's kernels, by phase:
+ * t[0] sampler, t[1] densities, t[2] estimator, t[3] block statistics,
+ * t[4] primitive pairs, t[5] analytic ERIs, t[6] copies, t[7] cleanup;
+ * n[k] = number of launches of that phase.  Arrays of 8. */
+int zv_last_timings(const zv_context *ctx, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZVMC_H */

@@ -1,0 +1,136 @@
+// Internal declarations of the ZVMC HIP library (context, device buffers,
+// kernel launchers).  Host code and kernels are compiled for gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/zvmc.h"
+
+#define ZV_GMAX 32     // max Gaussians per STO fit handled on the device
+#define ZV_NW 40       // walker block of the reference estimator (src/common_data.F90:4)
+#define ZV_TILE 64     // pair tile of the fast estimator
+#define ZV_SUB 32      // samples per LDS stage of the fast estimator
+#define ZV_LMAX_T 4    // largest pair angular momentum with a templated ERI kernel
+#define ZV_LPAIR_MAX 8 // largest pair angular momentum supported at all
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Phase { PH_SAMPLE = 0, PH_DENS, PH_EST, PH_STATS, PH_PRIM, PH_ERI, PH_COPY, PH_CLEAN, PH_N };
+
+struct zv_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // basis (host)
+    int n = 0;
+    int npair = 0;
+    int npair_pad = 0;
+    int64_t nint = 0;
+    double a_zv = 0.0;
+    std::vector<int> h_npower, h_nsto, h_itype, h_ngauss, h_lao;
+    std::vector<double> h_center, h_zeta, h_c1, h_c2, h_g;
+    std::vector<int> h_pair_i, h_pair_k;   // 1-based AO indices of pair p
+    std::vector<int> h_pair_L;             // Hermite order of pair p
+    std::vector<int64_t> h_pp_off;         // primitive-pair slot offset of pair p
+    std::vector<int64_t> h_e_off;          // Hermite-coefficient offset of pair p
+    int64_t pp_total = 0, e_total = 0;
+
+    // basis (device)
+    double *d_center = nullptr, *d_zeta = nullptr, *d_c1 = nullptr, *d_c2 = nullptr,
+           *d_g = nullptr;
+    int *d_npower = nullptr, *d_nsto = nullptr, *d_itype = nullptr, *d_ngauss = nullptr;
+    int *d_pair_i = nullptr, *d_pair_k = nullptr;
+    double *d_pair_f = nullptr, *d_pair_jf = nullptr, *d_pair_G = nullptr; // G: [3][npair_pad]
+    int64_t *d_kbase = nullptr;
+
+    // work buffers, grown on demand
+    DevBuf b_r1, b_r2, b_pi0;          // samples of one chunk
+    DevBuf b_pd;                       // per (pair, sample) data of one chunk
+    DevBuf b_accS, b_accG;             // per quartet block accumulators (triangle order)
+    DevBuf b_s1, b_s2;                 // per quartet sums over blocks
+    DevBuf b_out, b_out2;              // reference-order outputs
+    DevBuf b_blk;                      // optional block means
+    DevBuf b_pp;                       // primitive-pair data
+    DevBuf b_ppE;                      // primitive-pair Hermite coefficients
+    DevBuf b_ppcnt;                    // kept primitive pairs per AO pair
+    DevBuf b_tri;                      // triangle-order ERIs
+    DevBuf b_list;                     // class lists
+    DevBuf b_part;                     // split partial sums
+    DevBuf b_misc;
+    bool pp_ready = false;
+
+    // timings of the last call: events recorded around each launch group
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> pend_phase;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend_ev;
+    double t_ms[PH_N] = {};
+    int64_t t_n[PH_N] = {};
+};
+
+// ---- helpers (zvmc_api.hip) -------------------------------------------
+int zv_fail(zv_context *c, int code, const char *fmt, ...);
+int zv_hip(zv_context *c, hipError_t e, const char *what);
+int zv_ensure(zv_context *c, DevBuf &b, size_t bytes);
+
+// triangle index of (P, Q), P >= Q (0-based pair indices)
+__host__ __device__ inline int64_t zv_tri(int64_t P, int64_t Q) { return P * (P + 1) / 2 + Q; }
+
+// 0-based pair index of (i, k), i >= k, 1-based AO indices
+__host__ __device__ inline int zv_pidx(int i, int k) { return (i * (i - 1)) / 2 + (k - 1); }
+
+// kcp (0-based) of canonical quartet (ik|jl) from the base table
+__host__ __device__ inline int64_t zv_kcp(const int64_t *kbase, int n, int i, int j, int k, int l)
+{
+    int start = (k > j ? k : j) + ((l > k && j >= k) ? 1 : 0);
+    return kbase[((int64_t)(l - 1) * n + (k - 1)) * n + (j - 1)] + (i - start);
+}
+
+// ---- timing ----------------------------------------------------------------
+// Records a start event now and a stop event when it goes out of scope; the
+// elapsed times are summed per phase by zv_collect_timings after a sync.
+struct PhaseTimer {
+    zv_context *c;
+    Phase ph;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseTimer(zv_context *c_, Phase p);
+    ~PhaseTimer();
+};
+void zv_reset_timings(zv_context *c);
+int zv_collect_timings(zv_context *c);
+
+// ---- launchers (k_mc.hip, k_eri.hip, k_clean.hip) ---------------------
+int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uint64_t stream,
+              int precision, const double *r1, const double *r2, const double *pi0,
+              double *mean, double *err, double *block_means);
+int zv_run_draw(zv_context *c, uint64_t seed, uint64_t stream, int64_t m0, int64_t m,
+                double *r1, double *r2, double *pi0);
+int zv_run_pair_samples(zv_context *c, int i, int k, int ns, const double *r1, const double *r2,
+                        double *ut1, double *ut2, double *rho1, double *rho2, double *rhog1,
+                        double *rhog2, double *jac1, double *jac2);
+int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_screened);
+int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, const int *qj,
+                      const int *ql, double *out);
+int zv_run_scatter_ref(zv_context *c, const double *d_tri, double *host_out);
+int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri, int *n_removed);
+int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed);
+
+#define ZV_CHECK(call)                                                                            \
+    do {                                                                                          \
+        int rc_ = (call);                                                                         \
+        if (rc_ != ZV_OK) return rc_;                                                             \
+    } while (0)
+#define ZV_HIPCHECK(c, call)                                                                      \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return zv_hip((c), e_, #call);                                      \
+    } while (0)
