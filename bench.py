@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Time to a 1e-6 Ha standard error on every unique (ik|jl): the ZVMC ERI job.
+
+One step is the whole two-electron job of the reference's two_e_slater for a
+seeded 12-atom molecule (5 heavy atoms with 1S,2S,2S,2P,2P,3D; 7 H with
+1S,1S,2P; zeta ~ U[0.8, 8]; 110 STO orbitals, 6105 pairs, 18,638,565
+quartets; 20 Gaussians per STO fit):
+
+  1. analytic Gaussian-fit ERIs with the Cauchy-Schwarz driver (FP64),
+  2. a pilot of 10 blocks x 4000 Monte Carlo samples,
+  3. if the largest standard error is above 1e-6 Ha, a production run of 10
+     blocks sized from the pilot (stderr ~ 1/sqrt(samples)).
+
+The timed region starts with the basis resident on the device and ends with
+the ERIs, means and standard errors back on the host.  FP64 is the plain
+GPU form (``value``); the FP32 estimator variant is reported beside it.
+A bounded sample of the same job on the CPU oracle (all host cores) gives
+``cpu_baseline``.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "sec to 1e-6 Ha stderr on all (ij|kl), mixed-prec vs FP64 GPU vs CPU; MFMA %"
+TARGET = 1e-6
+NBL = 10
+PILOT = 4000
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
+
+
+def job(eng, bas, prec, seed):
+    eng.set_basis(bas)  # inputs resident; clears the primitive-pair cache
+    t0 = time.perf_counter()
+    eri, pre, nscr = eng.gauss_eri()
+    t1 = time.perf_counter()
+    eri_stats = eng.eri_stats()
+    an_timings = eng.timings()
+    mean, err = eng.mc_blocks(NBL, PILOT, prec, seed=seed, stream=0)
+    samples = NBL * PILOT
+    emax = float(err.max())
+    npts = PILOT
+    mc_timings = eng.timings()
+    if emax > TARGET:
+        npts = int(math.ceil(PILOT * (emax / TARGET) ** 2 * 1.2 / 40.0)) * 40
+        npts = min(npts, 1_000_000)
+        mean, err = eng.mc_blocks(NBL, npts, prec, seed=seed, stream=1)
+        samples += NBL * npts
+        emax = float(err.max())
+        mc_timings = eng.timings()
+    t2 = time.perf_counter()
+    return dict(total=t2 - t0, analytic=t1 - t0, mc=t2 - t1, max_stderr=emax, samples=samples,
+                production_npts=npts, n_screened=nscr, eri_stats=eri_stats,
+                an_timings=an_timings, mc_timings=mc_timings, eri=eri, mean=mean)
+
+
+def roofline_of(res):
+    """The heaviest ERI class kernel of the last step: algorithmic FP64 flops
+    (primitive quartets x flops per primitive quartet of its class) over its
+    measured kernel time (HIP events on the library's stream)."""
+    st = res["eri_stats"]
+    (lb, lk), d = max(st.items(), key=lambda kv: kv[1]["ms"])
+    sec = d["ms"] / 1e3
+    flops = d["prim_quartets"] * d["flops_per_pq"]
+    achieved = flops / sec / 1e12
+    traffic = None
+    tf = ROOT / "profiles" / "eri_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get(f"k_eri_class<{lb}, {lk}>")
+        except Exception:
+            traffic = None
+    total_flops = sum(v["prim_quartets"] * v["flops_per_pq"] for v in st.values())
+    total_ms = sum(v["ms"] for v in st.values())
+    return {"bound": "fp64-valu", "kernel": f"k_eri_class<{lb}, {lk}>", "achieved": achieved,
+            "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+            "traffic": traffic, "launches": d["launches"],
+            "avg_launch_ms": d["ms"] / max(1, d["launches"]),
+            "flops_per_launch": flops / max(1, d["launches"]),
+            "all_eri_classes_tflops": total_flops / (total_ms / 1e3) / 1e12}
+
+
+def cpu_baseline(bas, samples, nscr, budget_s=20.0):
+    """The CPU oracle (C restatement of the reference, OpenMP) on a bounded
+    sample of the same job, scaled to the whole job:
+      analytic: gauss_ijkl on a seeded random subset of quartets, scaled by the
+                number of unscreened quartets;
+      MC: one 40-sample block over all quartets, scaled by the job's samples."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle
+    from slater_stoch_amd import basis as B
+    orc = Oracle()
+    orc.set_basis(bas)
+    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    q = B.quartet_mapping(bas.n)
+    nint = q[0].size
+    rng = np.random.default_rng(77)
+    # analytic: grow the subset until the time budget is half used
+    done, t_an, nsub = 0, 0.0, 0
+    perm = rng.permutation(nint)
+    chunk = 64
+    while t_an < budget_s / 2 and done < nint:
+        pick = perm[done:done + chunk]
+        t = time.perf_counter()
+        orc.gauss_eri([a[pick] for a in q], None)
+        t_an += time.perf_counter() - t
+        done += pick.size
+        chunk = min(4 * chunk, 4096)
+    an_est = t_an / done * (nint - nscr)
+    # MC: one block of 40 samples over every quartet
+    r = np.random.default_rng(5)
+    r1, r2 = r.standard_normal((40, 3)), r.standard_normal((40, 3))
+    pi0 = orc.pi0(r1, r2)
+    t = time.perf_counter()
+    orc.mc_fixed(1, 40, r1, r2, pi0, q)
+    t_mc = time.perf_counter() - t
+    mc_est = t_mc * samples / 40.0
+    return {"value": an_est + mc_est, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"oracle gauss_ijkl on {done} random quartets ({t_an:.1f} s) scaled to "
+                       f"{nint - nscr} unscreened quartets; oracle estimator for all {nint} "
+                       f"quartets x 40 samples ({t_mc:.1f} s) scaled to {samples} samples"),
+            "analytic_s": an_est, "mc_s": mc_est}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-estimator variant")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from slater_stoch_amd import configs
+    from slater_stoch_amd.zvmc import Engine
+
+    bas = configs.random_molecule()
+    eng = Engine(0)
+    for w in range(args.warmup):
+        job(eng, bas, "fp64", seed=1000 + w)
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    res64 = []
+    for s in range(args.steps):
+        res64.append(job(eng, bas, "fp64", seed=1 + s))
+        r = res64[-1]
+        print(f"[bench] fp64 step {s + 1}/{args.steps}: {r['total']:.2f} s (analytic "
+              f"{r['analytic']:.2f} s, MC {r['mc']:.2f} s, {r['samples']} samples, max stderr "
+              f"{r['max_stderr']:.2e})", file=sys.stderr, flush=True)
+    t64 = float(np.mean([r["total"] for r in res64]))
+    res32 = None
+    if not args.no_fp32:
+        res32 = job(eng, bas, "fp32", seed=1)
+        print(f"[bench] fp32 step: {res32['total']:.2f} s (MC {res32['mc']:.2f} s)",
+              file=sys.stderr, flush=True)
+    last = res64[-1]
+    roof = roofline_of(last)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(bas, last["samples"], last["n_screened"])
+        print(f"[bench] cpu baseline estimate {cpu['value']:.0f} s on {cpu['cores']} cores",
+              file=sys.stderr, flush=True)
+    mct = last["mc_timings"]
+    est_ms = mct["estimator"][0]
+    line = {
+        "metric": METRIC,
+        "value": t64,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random molecule, device Philox samples)",
+        "config": {"workload": "random 12-atom molecule (5 C/N/O + 7 H), 110 STO AOs, "
+                               "6105 pairs, 18638565 quartets, 20-Gaussian fits; "
+                               "analytic FP64 ERIs + ZVMC to max stderr <= 1e-6 Ha, 10 blocks",
+                   "nint": int(bas.nint), "npair": int(bas.npair), "nbasis": int(bas.n),
+                   "blocks": NBL, "samples": int(last["samples"]),
+                   "max_stderr": last["max_stderr"], "estimator": "fp64"},
+        "analytic_s": float(np.mean([r["analytic"] for r in res64])),
+        "mc_s": float(np.mean([r["mc"] for r in res64])),
+        "mixed_precision_s": None if res32 is None else res32["total"],
+        "mixed_precision_max_stderr": None if res32 is None else res32["max_stderr"],
+        "estimator_qsamples_per_s": (bas.nint * NBL * last["production_npts"] / (est_ms / 1e3)
+                                     if est_ms > 0 else None),
+        "mfma_fraction": 0.0,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = cpu["value"] / t64
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,15 @@ int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
  * n[k] = number of launches of that phase.  Arrays of 8. */
 int zv_last_timings(const zv_context *ctx, double *ms, int64_t *launches);
 
+/* Per ERI class of the last zv_gauss_eri call (class index 6*LB + LK, LB >= LK
+ * the Hermite orders of the two pairs, 5 = any order above 4): kernel
+ * milliseconds, launches, primitive quartets evaluated and FP64 flops per
+ * primitive quartet of the class kernel.  Arrays of 36.  Diagnostic entry
+ * point with no counterpart in the reference (which prints only a CPU time,
+ * src/two_e_slater.F90:286-287). */
+int zv_last_eri_stats(const zv_context *ctx, double *ms, int64_t *launches,
+                      double *prim_quartets, double *flops_per_pq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,7 @@
 // primitive pairs.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "zvmc_internal.h"
 
@@ -373,6 +374,10 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// Loop order: each lane owns one ket primitive pair (its position, exponent,
+// coefficient and Hermite coefficients stay in registers); the bra primitive
+// pairs are walked in the inner loop with wave-uniform (scalar) loads, so the
+// hot loop issues no vector memory traffic.
 template <int LB, int LK>
 __global__ __launch_bounds__(256) void k_eri_class(PP pp, QSrc S, int nsplit, double *out,
                                                    double *part)
@@ -396,16 +401,22 @@ __global__ __launch_bounds__(256) void k_eri_class(PP pp, QSrc S, int nsplit, do
         int64_t ob = pp.off[Pb], ok = pp.off[Pk];
         int64_t eb = pp.eoff[Pb], ek = pp.eoff[Pk];
         int capb = pp.cap[Pb], capk = pp.cap[Pk];
-        for (int a = a0; a < a1; a++) {
-            double p = pp.p[ob + a], ip = pp.ip[ob + a];
-            double Px = pp.Px[ob + a], Py = pp.Py[ob + a], Pz = pp.Pz[ob + a];
-            double cb = pp.c[ob + a];
-            double Eb[HB];
+        for (int c0 = 0; c0 < nk; c0 += 64) {
+            int c = c0 + lane;
+            bool live = c < nk;
+            int cc = live ? c : c0; // in-range index for idle lanes
+            double q = pp.p[ok + cc], iq = pp.ip[ok + cc];
+            double Qx = pp.Px[ok + cc], Qy = pp.Py[ok + cc], Qz = pp.Pz[ok + cc];
+            double ck = live ? pp.c[ok + cc] : 0.0;
+            double Ek[HK];
 #pragma unroll
-            for (int h = 0; h < HB; h++) Eb[h] = pp.E[eb + (int64_t)h * capb + a];
-            for (int c = lane; c < nk; c += 64) {
-                double q = pp.p[ok + c], iq = pp.ip[ok + c];
-                double X = Px - pp.Px[ok + c], Y = Py - pp.Py[ok + c], Z = Pz - pp.Pz[ok + c];
+            for (int h = 0; h < HK; h++)
+                Ek[h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -pp.E[ek + (int64_t)h * capk + cc]
+                                                             : pp.E[ek + (int64_t)h * capk + cc];
+            for (int a = a0; a < a1; a++) {
+                double p = pp.p[ob + a], ip = pp.ip[ob + a];
+                double X = pp.Px[ob + a] - Qx, Y = pp.Py[ob + a] - Qy, Z = pp.Pz[ob + a] - Qz;
+                double cb = pp.c[ob + a];
                 double s = p + q;
                 double is = 1.0 / s;
                 double alpha = (p * q) * is;
@@ -419,17 +430,13 @@ __global__ __launch_bounds__(256) void k_eri_class(PP pp, QSrc S, int nsplit, do
                 for (int hb = 0; hb < HB; hb++) {
                     double w = 0.0;
 #pragma unroll
-                    for (int hk = 0; hk < HK; hk++) {
-                        double ekv = pp.E[ek + (int64_t)hk * capk + c];
-                        double r = R[hidx(kH.t[hb] + kH.t[hk], kH.u[hb] + kH.u[hk],
-                                          kH.v[hb] + kH.v[hk])];
-                        if ((kH.t[hk] + kH.u[hk] + kH.v[hk]) & 1) w = fma(-ekv, r, w);
-                        else w = fma(ekv, r, w);
-                    }
-                    sum = fma(Eb[hb], w, sum);
+                    for (int hk = 0; hk < HK; hk++)
+                        w = fma(Ek[hk], R[hidx(kH.t[hb] + kH.t[hk], kH.u[hb] + kH.u[hk],
+                                                kH.v[hb] + kH.v[hk])], w);
+                    sum = fma(pp.E[eb + (int64_t)hb * capb + a], w, sum);
                 }
                 double pref = ((kPi52 * ip) * iq) * sqrt(is);
-                acc = fma((cb * pp.c[ok + c]) * pref, sum, acc);
+                acc = fma((cb * ck) * pref, sum, acc);
             }
         }
     }
@@ -602,9 +609,35 @@ static int ensure_primpairs(zv_context *c, PP &pp)
                            c->d_pair_k, pp);
         ZV_HIPCHECK(c, hipGetLastError());
     }
+    c->h_cnt.assign(np, 0);
+    ZV_HIPCHECK(c, hipMemcpyAsync(c->h_cnt.data(), d_cnt, sizeof(int) * np, hipMemcpyDeviceToHost,
+                                  c->stream));
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->pp_ready = true;
     return ZV_OK;
+}
+
+// primitive quartets a class launch evaluates (screened quartets excluded)
+double class_pq(const zv_context *c, int mode, const std::vector<int> &lb,
+                       const std::vector<int> &lk, const std::vector<double> *pre)
+{
+    const std::vector<int> &cnt = c->h_cnt;
+    auto keep = [&](int P, int Q) {
+        return !pre || fabs((*pre)[P] * (*pre)[Q]) >= 1.0e-30;
+    };
+    double s = 0.0;
+    if (mode == Q_DIAG) {
+        for (int P : lb) s += (double)cnt[P] * cnt[P];
+    } else if (mode == Q_TRI) {
+        for (size_t x = 0; x < lb.size(); x++)
+            for (size_t y = 0; y < x; y++)
+                if (keep(lb[x], lb[y])) s += (double)cnt[lb[x]] * cnt[lb[y]];
+    } else {
+        for (int P : lb)
+            for (int Q : lk)
+                if (keep(P, Q)) s += (double)cnt[P] * cnt[Q];
+    }
+    return s;
 }
 
 // Launch one class over the quartet source S, splitting bra primitive pairs
@@ -621,7 +654,14 @@ static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, dou
         part = (double *)c->b_part.p;
     }
     int64_t items = S.nq * nsplit;
-    PhaseTimer tm(c, PH_ERI);
+    static const bool trace = getenv("ZV_TRACE") != nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (trace) {
+        hipEventCreate(&t0);
+        hipEventCreate(&t1);
+        hipEventRecord(t0, c->stream);
+    }
+    PhaseTimer tm(c, PH_ERI, LB * 6 + LK);
     if (LB <= ZV_LMAX_T && LK <= ZV_LMAX_T) {
         if (!launch_class_t(LB, LK, nblk(items * 64, 256), c->stream, pp, S, nsplit, out, part))
             return zv_fail(c, ZV_ERR_STATE, "no ERI kernel for class (%d,%d)", LB, LK);
@@ -634,6 +674,16 @@ static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, dou
         hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(S.nq, 256)), dim3(256), 0, c->stream, S,
                            nsplit, (const double *)part, out);
         ZV_HIPCHECK(c, hipGetLastError());
+    }
+    if (trace) {
+        hipEventRecord(t1, c->stream);
+        hipEventSynchronize(t1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t0, t1);
+        fprintf(stderr, "[zvmc] eri class (%d,%d) mode %d quartets %lld split %d: %.3f ms\n", LB,
+                LK, S.mode, (long long)S.nq, nsplit, ms);
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
     }
     return ZV_OK;
 }
@@ -681,6 +731,9 @@ int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_sc
     }
     hipLaunchKernelGGL(k_diag_pre, dim3(nblk(np, 256)), dim3(256), 0, c->stream, np, tri, pre);
     ZV_HIPCHECK(c, hipGetLastError());
+    std::vector<double> hpre(np);
+    ZV_HIPCHECK(c, hipMemcpyAsync(hpre.data(), pre, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                  c->stream));
     // phase 2: off-diagonal quartets with the Cauchy-Schwarz test
     for (int tb = 0; tb < ncls; tb++)
         for (int tk = 0; tk <= tb; tk++) {
@@ -695,8 +748,10 @@ int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_sc
             ZV_CHECK(launch_class(c, pp, S, order_of(tb), order_of(tk), tri));
         }
     ZV_CHECK(zv_run_scatter_ref(c, tri, ijkl));
-    std::vector<double> hpre(np);
-    ZV_HIPCHECK(c, hipMemcpy(hpre.data(), pre, sizeof(double) * np, hipMemcpyDeviceToHost));
+    // work accounting is done on request (zv_last_eri_stats)
+    c->last_lists = lists;
+    c->last_pre = hpre;
+    c->pq_pending = true;
     if (precond) {
         int n = c->n;
         for (int i = 1; i <= n; i++)
@@ -778,4 +833,21 @@ int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, c
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     for (int64_t t = 0; t < nq; t++) out[perm[t]] = h[t];
     return ZV_OK;
+}
+
+// per-class primitive-quartet counts of the last zv_gauss_eri (host work)
+void zv_eri_account(const zv_context *cc)
+{
+    zv_context *c = const_cast<zv_context *>(cc);
+    if (!c->pq_pending) return;
+    const auto &lists = c->last_lists;
+    int ncls = (int)lists.size();
+    auto order_of = [&](int t) { return t > ZV_LMAX_T ? ZV_LMAX_T + 1 : t; };
+    for (int t = 0; t < ncls; t++)
+        c->cls_pq[order_of(t) * 6 + order_of(t)] += class_pq(c, Q_DIAG, lists[t], lists[t], nullptr);
+    for (int tb = 0; tb < ncls; tb++)
+        for (int tk = 0; tk <= tb; tk++)
+            c->cls_pq[order_of(tb) * 6 + order_of(tk)] +=
+                class_pq(c, tb == tk ? Q_TRI : Q_RECT, lists[tb], lists[tk], &c->last_pre);
+    c->pq_pending = false;
 }

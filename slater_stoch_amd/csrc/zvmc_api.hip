@@ -45,7 +45,7 @@ int zv_ensure(zv_context *c, DevBuf &b, size_t bytes)
 }
 
 // ---------------------------------------------------------------- timing
-PhaseTimer::PhaseTimer(zv_context *c_, Phase p) : c(c_), ph(p)
+PhaseTimer::PhaseTimer(zv_context *c_, Phase p, int cls_) : c(c_), ph(p), cls(cls_)
 {
     auto take = [&]() -> hipEvent_t {
         if (c->ev_used == c->ev_pool.size()) {
@@ -65,6 +65,7 @@ PhaseTimer::~PhaseTimer()
     if (!a || !b) return;
     hipEventRecord(b, c->stream);
     c->pend_phase.push_back(ph);
+    c->pend_cls.push_back(cls);
     c->pend_ev.emplace_back(a, b);
 }
 
@@ -74,7 +75,14 @@ void zv_reset_timings(zv_context *c)
         c->t_ms[p] = 0.0;
         c->t_n[p] = 0;
     }
+    for (int k = 0; k < 36; k++) {
+        c->cls_ms[k] = 0.0;
+        c->cls_n[k] = 0;
+        c->cls_pq[k] = 0.0;
+    }
+    c->pq_pending = false;
     c->pend_phase.clear();
+    c->pend_cls.clear();
     c->pend_ev.clear();
     c->ev_used = 0;
 }
@@ -87,9 +95,15 @@ int zv_collect_timings(zv_context *c)
         if (hipEventElapsedTime(&ms, c->pend_ev[t].first, c->pend_ev[t].second) == hipSuccess) {
             c->t_ms[c->pend_phase[t]] += ms;
             c->t_n[c->pend_phase[t]] += 1;
+            int k = c->pend_cls[t];
+            if (k >= 0 && k < 36) {
+                c->cls_ms[k] += ms;
+                c->cls_n[k] += 1;
+            }
         }
     }
     c->pend_phase.clear();
+    c->pend_cls.clear();
     c->pend_ev.clear();
     c->ev_used = 0;
     return ZV_OK;
@@ -213,6 +227,33 @@ extern "C" int zv_last_timings(const zv_context *c, double *ms, int64_t *launche
     for (int p = 0; p < PH_N; p++) {
         if (ms) ms[p] = c->t_ms[p];
         if (launches) launches[p] = c->t_n[p];
+    }
+    return ZV_OK;
+}
+
+// FP64 flops per primitive quartet of the (LB, LK) class kernel: Boys values
+// (3 per order plus ~25 for erf/exp/sqrt/division), the Hermite R recursion
+// (2 FMA per entry), the ket-Hermite contraction (one FMA per pair of
+// Hermite indices), the bra sum (one FMA per bra index) and the prefactor.
+static double eri_flops_per_pq(int LB, int LK)
+{
+    int L = LB + LK;
+    double HB = (LB + 1) * (LB + 2) * (LB + 3) / 6.0, HK = (LK + 1) * (LK + 2) * (LK + 3) / 6.0;
+    double HR4 = (L + 1) * (L + 2) * (L + 3) * (L + 4) / 24.0; // R entries over all levels
+    return 25.0 + 3.0 * (L + 1) + 4.0 * (HR4 - 1.0) + 2.0 * HB * HK + 2.0 * HB + 12.0;
+}
+
+extern "C" int zv_last_eri_stats(const zv_context *c, double *ms, int64_t *launches,
+                                 double *prim_quartets, double *flops_per_pq)
+{
+    if (!c) return ZV_ERR_ARG;
+    zv_eri_account(c);
+    for (int k = 0; k < 36; k++) {
+        int LB = k / 6, LK = k % 6;
+        if (ms) ms[k] = c->cls_ms[k];
+        if (launches) launches[k] = c->cls_n[k];
+        if (prim_quartets) prim_quartets[k] = c->cls_pq[k];
+        if (flops_per_pq) flops_per_pq[k] = (LB < 5 && LK < 5) ? eri_flops_per_pq(LB, LK) : 0.0;
     }
     return ZV_OK;
 }

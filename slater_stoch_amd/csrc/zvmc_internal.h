@@ -75,7 +75,16 @@ struct zv_context {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pend_ev;
     double t_ms[PH_N] = {};
     int64_t t_n[PH_N] = {};
+    std::vector<int> pend_cls;        // ERI class of each pending timer (-1: none)
+    double cls_ms[36] = {};
+    int64_t cls_n[36] = {};
+    double cls_pq[36] = {};           // primitive quartets evaluated per class
+    std::vector<int> h_cnt;           // kept primitive pairs per AO pair (host copy)
+    std::vector<std::vector<int>> last_lists; // pair lists by Hermite order, last run
+    std::vector<double> last_pre;     // Schwarz diagonal of the last run
+    bool pq_pending = false;
 };
+void zv_eri_account(const zv_context *c);
 
 // ---- helpers (zvmc_api.hip) -------------------------------------------
 int zv_fail(zv_context *c, int code, const char *fmt, ...);
@@ -101,8 +110,9 @@ __host__ __device__ inline int64_t zv_kcp(const int64_t *kbase, int n, int i, in
 struct PhaseTimer {
     zv_context *c;
     Phase ph;
+    int cls;
     hipEvent_t a = nullptr, b = nullptr;
-    PhaseTimer(zv_context *c_, Phase p);
+    PhaseTimer(zv_context *c_, Phase p, int cls_ = -1);
     ~PhaseTimer();
 };
 void zv_reset_timings(zv_context *c);

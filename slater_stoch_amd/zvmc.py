@@ -43,6 +43,7 @@ SIGNATURES = {
     "zv_svd_clean": (I32, [VP, I32, I32, U64, VP, VP]),
     "zv_davidson_clean": (I32, [VP, VP, VP]),
     "zv_last_timings": (I32, [VP, VP, VP]),
+    "zv_last_eri_stats": (I32, [VP, VP, VP, VP, VP]),
 }
 
 _LIB = None
@@ -199,6 +200,19 @@ class Engine:
         nrem = np.zeros(1, np.int32)
         self._check(self.lib.zv_davidson_clean(self.h, _p(e), _p(nrem)))
         return e, int(nrem[0])
+
+    def eri_stats(self):
+        """Per (LB, LK) class of the last gauss_eri: ms, launches, primitive
+        quartets, FP64 flops per primitive quartet."""
+        ms, pq, fl = np.zeros(36), np.zeros(36), np.zeros(36)
+        n = np.zeros(36, np.int64)
+        self._check(self.lib.zv_last_eri_stats(self.h, _p(ms), _p(n), _p(pq), _p(fl)))
+        out = {}
+        for k in range(36):
+            if n[k] or pq[k]:
+                out[(k // 6, k % 6)] = dict(ms=float(ms[k]), launches=int(n[k]),
+                                            prim_quartets=float(pq[k]), flops_per_pq=float(fl[k]))
+        return out
 
     def timings(self):
         ms = np.zeros(8)
