@@ -1,14 +1,382 @@
-// PSD cleanup of the ERI matrix (src/svd_clean.F90, src/davidson_clean.F90).
+// PSD cleanup of the ERI matrix on the GPU.
+//
+// Both cleanups act on the reference's product-basis matrix
+// W((i,j),(k,l)) = <ij|kl>, N = n^2, column-major W(ii,jj) with
+// ii = i + (j-1) n, jj = k + (l-1) n, filled from the unique quartets through
+// the eight permutations of src/svd_clean.F90:97-130.
+//
+// zv_svd_clean follows src/svd_clean.F90:143-367: passes of a randomized
+// range finder (Gaussian test matrix, Z = W_work P, q power iterations with QR
+// re-orthonormalization), SVD of Y = Z^T W_work, U = Z U_Y; a singular triple
+// with u.v < 0 is a negative eigenvalue, |u.v| < 0.99 an unconverged one;
+// converged triples are deflated from W_work and negative ones removed from
+// W; passes stop when N vectors are accepted or max|D| < 1e-12.  Finally
+// moy = (1/8) sum of the eight permutations (src/svd_clean.F90:372-405).
+// GEMMs are rocBLAS DGEMM, QR rocSOLVER geqrf/orgqr, SVD rocSOLVER gesvd.
+//
+// zv_davidson_clean is the dense branch of src/davidson_clean.F90:30-120:
+// symmetric eigendecomposition (rocSOLVER syevd) and W -= sum_{e<0} e u u^T.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+
 #include "zvmc_internal.h"
+
+namespace {
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ inline void tri_decode(int64_t t, int64_t &P, int64_t &Q)
+{
+    int64_t p = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (p * (p + 1) / 2 > t) p--;
+    while ((p + 1) * (p + 2) / 2 <= t) p++;
+    P = p;
+    Q = t - p * (p + 1) / 2;
+}
+
+// (i,j,k,l) of the eight permutations, src/svd_clean.F90:99-128
+__device__ inline void perm8(int s, int i, int j, int k, int l, int &a, int &b, int &c, int &d)
+{
+    switch (s) {
+    case 0: a = i; b = j; c = k; d = l; break;
+    case 1: a = k; b = j; c = i; d = l; break;
+    case 2: a = k; b = l; c = i; d = j; break;
+    case 3: a = i; b = l; c = k; d = j; break;
+    case 4: a = j; b = i; c = l; d = k; break;
+    case 5: a = j; b = k; c = l; d = i; break;
+    case 6: a = l; b = k; c = j; d = i; break;
+    default: a = l; b = i; c = j; d = k; break;
+    }
+}
+
+// quartet t of the pair triangle -> reference (i,j,k,l) and its kcp
+__device__ inline int64_t quartet_ref(int64_t t, const int *pi, const int *pk,
+                                      const int64_t *kbase, int n, int &i, int &j, int &k, int &l)
+{
+    int64_t P, Q;
+    tri_decode(t, P, Q);
+    i = pi[P];
+    k = pk[P];
+    j = pi[Q];
+    l = pk[Q];
+    return zv_kcp(kbase, n, i, j, k, l);
+}
+
+__global__ void k_build_W(int64_t ntri, const double *eri, double *W, int n, const int *pi,
+                          const int *pk, const int64_t *kbase)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    int i, j, k, l;
+    int64_t kcp = quartet_ref(t, pi, pk, kbase, n, i, j, k, l);
+    double v = eri[kcp];
+    int64_t N = (int64_t)n * n;
+    for (int s = 0; s < 8; s++) {
+        int a, b, c, d;
+        perm8(s, i, j, k, l, a, b, c, d);
+        int64_t ii = (a - 1) + (int64_t)(b - 1) * n, jj = (c - 1) + (int64_t)(d - 1) * n;
+        W[ii + jj * N] = v;
+    }
+}
+
+// moy(kcp): average of the eight permutations (svd_clean) or the first one
+// (davidson_clean, src/davidson_clean.F90:110-114)
+__global__ void k_gather(int64_t ntri, const double *W, double *eri, int n, const int *pi,
+                         const int *pk, const int64_t *kbase, int avg8)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    int i, j, k, l;
+    int64_t kcp = quartet_ref(t, pi, pk, kbase, n, i, j, k, l);
+    int64_t N = (int64_t)n * n;
+    double m = 0.0;
+    int ns = avg8 ? 8 : 1;
+    for (int s = 0; s < ns; s++) {
+        int a, b, c, d;
+        perm8(s, i, j, k, l, a, b, c, d);
+        int64_t ii = (a - 1) + (int64_t)(b - 1) * n, jj = (c - 1) + (int64_t)(d - 1) * n;
+        m = avg8 ? m + W[ii + jj * N] * 0.125 : W[ii + jj * N];
+    }
+    eri[kcp] = m;
+}
+
+__global__ void k_fill0(double *p, int64_t count)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0.0;
+}
+
+// Philox4x32-10 normal deviates (Box-Muller, cosine branch as the reference,
+// src/svd_clean.F90:161-167), column k of an N x r matrix
+__device__ inline void philox(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                              uint32_t k1)
+{
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__global__ void k_randn(int64_t total, uint64_t seed, uint32_t pass, double *P)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    uint32_t c0 = (uint32_t)t, c1 = (uint32_t)(t >> 32), c2 = pass, c3 = 0x5c1ea2u;
+    philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u1 = ((double)(((uint64_t)(c0 >> 5) << 26) | (c1 >> 6)) + 0.5) * 0x1.0p-53;
+    double u2 = ((double)(((uint64_t)(c2 >> 5) << 26) | (c3 >> 6)) + 0.5) * 0x1.0p-53;
+    P[t] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586477 * u2);
+}
+
+// one workgroup per column: x(:,k) /= ||x(:,k)||
+__global__ void k_colnormalize(int64_t N, double *X)
+{
+    __shared__ double red[256];
+    double *x = X + (int64_t)blockIdx.x * N;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < N; t += 256) s += x[t] * x[t];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    double inv = 1.0 / sqrt(red[0]);
+    for (int64_t t = threadIdx.x; t < N; t += 256) x[t] *= inv;
+}
+
+// dots[k] = U(:,k) . Vt(k,:)   (U N x r ld N, Vt r x N ld r)
+__global__ void k_coldot(int64_t N, int r, const double *U, const double *Vt, double *dots)
+{
+    __shared__ double red[256];
+    int k = blockIdx.x;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < N; t += 256) s += U[t + (int64_t)k * N] * Vt[k + t * r];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dots[k] = red[0];
+}
+
+// out(:,k) = U(:,k) * f(D_k): mode 0 |D|, mode 1 min(D, 0)
+__global__ void k_colscale(int64_t N, int r, const double *U, const double *D, double *out,
+                           int mode)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * r) return;
+    int k = (int)(t / N);
+    double d = D[k];
+    double f = mode == 0 ? fabs(d) : (d < 0.0 ? d : 0.0);
+    out[t] = U[t] * f;
+}
+
+} // namespace
+
+// one rocBLAS handle per context, bound to the context's stream
+static int blas_handle(zv_context *c, rocblas_handle *h)
+{
+    if (!c->blas) {
+        rocblas_handle nh;
+        if (rocblas_create_handle(&nh) != rocblas_status_success)
+            return zv_fail(c, ZV_ERR_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(nh, c->stream);
+        c->blas = nh;
+    }
+    *h = (rocblas_handle)c->blas;
+    return ZV_OK;
+}
+
+void zv_blas_destroy(zv_context *c)
+{
+    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
+    c->blas = nullptr;
+}
+
+#define RB(c, call)                                                                               \
+    do {                                                                                          \
+        rocblas_status s_ = (call);                                                               \
+        if (s_ != rocblas_status_success)                                                         \
+            return zv_fail((c), ZV_ERR_HIP, "%s failed: status %d", #call, (int)s_);              \
+    } while (0)
+
+// W (N x N) from the eri list in kcp order (host), 8 permutations
+static int build_W(zv_context *c, const double *eri_host, double *d_eri, double *W)
+{
+    int64_t ntri = c->nint, N = (int64_t)c->n * c->n;
+    ZV_HIPCHECK(c, hipMemcpyAsync(d_eri, eri_host, sizeof(double) * ntri, hipMemcpyHostToDevice,
+                                  c->stream));
+    hipLaunchKernelGGL(k_fill0, dim3(nblk(N * N, 256)), dim3(256), 0, c->stream, W, N * N);
+    hipLaunchKernelGGL(k_build_W, dim3(nblk(ntri, 256)), dim3(256), 0, c->stream, ntri,
+                       (const double *)d_eri, W, c->n, c->d_pair_i, c->d_pair_k, c->d_kbase);
+    ZV_HIPCHECK(c, hipGetLastError());
+    return ZV_OK;
+}
+
+static int qr_inplace(zv_context *c, rocblas_handle h, double *Z, int64_t N, int r, double *tau)
+{
+    RB(c, rocsolver_dgeqrf(h, (rocblas_int)N, r, Z, (rocblas_int)N, tau));
+    RB(c, rocsolver_dorgqr(h, (rocblas_int)N, r, r, Z, (rocblas_int)N, tau));
+    return ZV_OK;
+}
 
 int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri, int *n_removed)
 {
-    (void)rank; (void)q; (void)seed; (void)eri; (void)n_removed;
-    return zv_fail(c, ZV_ERR_STATE, "zv_svd_clean: not built in this version");
+    const int64_t N = (int64_t)c->n * c->n;
+    if (N > 65536) return zv_fail(c, ZV_ERR_ARG, "svd_clean: n^2 = %lld too large", (long long)N);
+    const int r = (int)std::min<int64_t>(rank, N);
+    rocblas_handle h;
+    ZV_CHECK(blas_handle(c, &h));
+    PhaseTimer tm(c, PH_CLEAN);
+    // workspace: W, Wwork (N^2), P, Z, U, S (N r), Y/Vt (r N), UY (r^2), small vectors
+    size_t nn = (size_t)N * N, nr = (size_t)N * r;
+    size_t words = 2 * nn + 4 * nr + 2 * nr + (size_t)r * r + 8 * (size_t)r + c->nint + 16;
+    ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
+    double *W = (double *)c->b_misc.p, *Ww = W + nn, *P = Ww + nn, *Z = P + nr, *U = Z + nr,
+           *S = U + nr, *Y = S + nr, *Vt = Y + nr, *UY = Vt + nr, *D = UY + (size_t)r * r,
+           *E = D + r, *tau = E + r, *dots = tau + r, *d_eri = dots + r + 4;
+    rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    ZV_CHECK(build_W(c, eri, d_eri, Ww));
+    ZV_HIPCHECK(c, hipMemcpyAsync(W, Ww, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    int64_t npass = 0;
+    int removed = 0;
+    std::vector<double> hD(r), hdots(r);
+    // the reference loops until N vectors are accepted; every pass either
+    // deflates a converged triple or ends the loop through max|D| < 1e-12, the
+    // bound below only guards against a pathological spectrum
+    const uint32_t max_pass = (uint32_t)(4 * (N / r) + 16);
+    for (uint32_t pass = 0; npass < N; pass++) {
+        if (pass >= max_pass)
+            return zv_fail(c, ZV_ERR_STATE, "svd_clean: no convergence after %u passes", pass);
+        hipLaunchKernelGGL(k_randn, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, (int64_t)nr,
+                           seed, pass, P);
+        hipLaunchKernelGGL(k_colnormalize, dim3(r), dim3(256), 0, c->stream, N, P);
+        ZV_HIPCHECK(c, hipGetLastError());
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, N, &one, Ww, N,
+                            P, N, &zero, Z, N));
+        ZV_CHECK(qr_inplace(c, h, Z, N, r, tau));
+        for (int it = 0; it < q; it++) {
+            RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, N, r, N,
+                                &one, Ww, N, Z, N, &zero, P, N));
+            RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, N, &one,
+                                Ww, N, P, N, &zero, Z, N));
+            ZV_CHECK(qr_inplace(c, h, Z, N, r, tau));
+        }
+        // Y (r x N) = Z^T W_work
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, N, N, &one,
+                            Z, N, Ww, N, &zero, Y, r));
+        RB(c, rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, r, (rocblas_int)N,
+                               Y, r, D, UY, r, Vt, r, E, rocblas_outofplace, info));
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, r, &one, Z, N,
+                            UY, r, &zero, U, N));
+        hipLaunchKernelGGL(k_coldot, dim3(r), dim3(256), 0, c->stream, N, r, U, Vt, dots);
+        ZV_HIPCHECK(c, hipGetLastError());
+        ZV_HIPCHECK(c, hipMemcpyAsync(hD.data(), D, sizeof(double) * r, hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipMemcpyAsync(hdots.data(), dots, sizeof(double) * r,
+                                      hipMemcpyDeviceToHost, c->stream));
+        rocblas_int hinfo = 0;
+        ZV_HIPCHECK(c, hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (hinfo != 0) return zv_fail(c, ZV_ERR_HIP, "%d: SVD failed", (int)hinfo);
+        // src/svd_clean.F90:286-296
+        npass += r;
+        double dmax = 0.0;
+        for (int k = 0; k < r; k++) {
+            if (hdots[k] < 0.0) hD[k] = -hD[k];
+            if (fabs(hdots[k]) < 0.99) {
+                hD[k] = 0.0;
+                npass -= 1;
+            }
+            dmax = std::max(dmax, fabs(hD[k]));
+            if (hD[k] < 0.0) removed++;
+        }
+        ZV_HIPCHECK(c, hipMemcpyAsync(D, hD.data(), sizeof(double) * r, hipMemcpyHostToDevice,
+                                      c->stream));
+        // W_work -= U |D| Vt ; W += U min(D,0) Vt  (src/svd_clean.F90:334-353)
+        hipLaunchKernelGGL(k_colscale, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, N, r, U, D,
+                           S, 0);
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, N, r, &mone, S,
+                            N, Vt, r, &one, Ww, N));
+        hipLaunchKernelGGL(k_colscale, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, N, r, U, D,
+                           S, 1);
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, N, r, &one, S,
+                            N, Vt, r, &one, W, N));
+        ZV_HIPCHECK(c, hipGetLastError());
+        if (dmax < 1.0e-12) break; // src/svd_clean.F90:359
+    }
+    hipLaunchKernelGGL(k_gather, dim3(nblk(c->nint, 256)), dim3(256), 0, c->stream, c->nint,
+                       (const double *)W, d_eri, c->n, c->d_pair_i, c->d_pair_k, c->d_kbase, 1);
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_HIPCHECK(c, hipMemcpyAsync(eri, d_eri, sizeof(double) * c->nint, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (n_removed) *n_removed = removed;
+    return ZV_OK;
 }
 
 int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed)
 {
-    (void)eri; (void)n_removed;
-    return zv_fail(c, ZV_ERR_STATE, "zv_davidson_clean: not built in this version");
+    const int64_t N = (int64_t)c->n * c->n;
+    if (N > 65536) return zv_fail(c, ZV_ERR_ARG, "davidson_clean: n^2 = %lld too large",
+                                  (long long)N);
+    rocblas_handle h;
+    ZV_CHECK(blas_handle(c, &h));
+    PhaseTimer tm(c, PH_CLEAN);
+    size_t nn = (size_t)N * N;
+    size_t words = 2 * nn + 2 * (size_t)N + c->nint + 16;
+    ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
+    double *W = (double *)c->b_misc.p, *U = W + nn, *ev = U + nn, *E = ev + N, *d_eri = E + N;
+    rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    ZV_CHECK(build_W(c, eri, d_eri, W));
+    ZV_HIPCHECK(c, hipMemcpyAsync(U, W, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
+    RB(c, rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)N, U,
+                           (rocblas_int)N, ev, E, info));
+    std::vector<double> hev(N);
+    rocblas_int hinfo = 0;
+    ZV_HIPCHECK(c, hipMemcpyAsync(hev.data(), ev, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (hinfo != 0) return zv_fail(c, ZV_ERR_HIP, "info=%d: DSYGV Diagonalization failed", hinfo);
+    int l = 0; // eigenvalues ascend: the first l are negative (src/davidson_clean.F90:92-96)
+    while (l < N && hev[l] < 0.0) l++;
+    if (l > 0) {
+        // W -= U_l diag(e_l) U_l^T, written as W += (U_l diag(-e_l)) U_l^T
+        std::vector<double> hd(l);
+        for (int k = 0; k < l; k++) hd[k] = -hev[k];
+        ZV_HIPCHECK(c, hipMemcpyAsync(E, hd.data(), sizeof(double) * l, hipMemcpyHostToDevice,
+                                      c->stream));
+        const double one = 1.0;
+        ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * (size_t)N * l));
+        double *Sl = (double *)c->b_part.p;
+        hipLaunchKernelGGL(k_colscale, dim3(nblk((int64_t)N * l, 256)), dim3(256), 0, c->stream,
+                           N, l, U, E, Sl, 0);
+        ZV_HIPCHECK(c, hipGetLastError());
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, N, N, l, &one,
+                            Sl, N, U, N, &one, W, N));
+    }
+    hipLaunchKernelGGL(k_gather, dim3(nblk(c->nint, 256)), dim3(256), 0, c->stream, c->nint,
+                       (const double *)W, d_eri, c->n, c->d_pair_i, c->d_pair_k, c->d_kbase, 0);
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_HIPCHECK(c, hipMemcpyAsync(eri, d_eri, sizeof(double) * c->nint, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (n_removed) *n_removed = l;
+    return ZV_OK;
 }

@@ -83,7 +83,9 @@ struct zv_context {
     std::vector<std::vector<int>> last_lists; // pair lists by Hermite order, last run
     std::vector<double> last_pre;     // Schwarz diagonal of the last run
     bool pq_pending = false;
+    void *blas = nullptr;             // rocBLAS handle (cleanup GEMMs)
 };
+void zv_blas_destroy(zv_context *c);
 void zv_eri_account(const zv_context *c);
 
 // ---- helpers (zvmc_api.hip) -------------------------------------------
