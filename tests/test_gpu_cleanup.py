@@ -7,8 +7,11 @@ by what the mathematics fixes:
   with numpy.linalg.eigh on the same product-basis matrix;
 * svd_clean's randomized passes converge to the same projection, averaged
   over the eight permutations;
-* a positive semidefinite input comes back unchanged and every output
-  matrix is positive semidefinite.
+* a positive semidefinite input comes back unchanged.
+The final average over the eight permutations (src/svd_clean.F90:372-405)
+mixes row and column indices (a partial transpose of W), so the cleaned list
+itself need not be positive semidefinite; the tests compare against the same
+averaged projection instead.
 """
 import numpy as np
 import pytest
@@ -80,11 +83,9 @@ def test_svd_clean_equals_projection_full_rank(engine):
     engine.set_basis(bas)
     got, nrem = engine.svd_clean(vals, rank=bas.n ** 2, q=12, seed=3)
     Wc, nneg = negative_removed(product_matrix(bas.n, q, vals))
-    assert nrem > 0
+    assert nrem == nneg > 0
     ref = gather(bas.n, q, Wc, True)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
-    Wout = product_matrix(bas.n, q, got)
-    assert np.linalg.eigvalsh(Wout).min() > -1e-9 * np.abs(ref).max()
 
 
 def test_svd_clean_leaves_psd_input(engine):
@@ -106,9 +107,11 @@ def test_svd_clean_partial_rank_passes(engine):
     eri, _, _ = engine.gauss_eri()
     q = B.quartet_mapping(bas.n)
     rng = np.random.default_rng(8)
-    vals = eri + 1e-3 * rng.standard_normal(eri.size)
+    # small noise: negative eigenvalues of order 1e-7, resolved by the passes
+    vals = eri + 1e-8 * rng.standard_normal(eri.size)
     got, nrem = engine.svd_clean(vals, rank=96, q=12, seed=5)
     Wc, nneg = negative_removed(product_matrix(bas.n, q, vals))
     ref = gather(bas.n, q, Wc, True)
     assert nrem > 0
-    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-7 * scale)
