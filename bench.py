@@ -62,29 +62,36 @@ def job(eng, bas, prec, seed):
                 an_timings=an_timings, mc_timings=mc_timings, eri=eri, mean=mean)
 
 
+def kernel_name(key):
+    kind, a, b = key
+    return f"k_eri_shell<{a}, {b}>" if kind == "shell" else (
+        f"k_eri_class<{a}, {b}>" if a < 5 else "k_eri_generic")
+
+
 def roofline_of(res):
-    """The heaviest ERI class kernel of the last step: algorithmic FP64 flops
-    (primitive quartets x flops per primitive quartet of its class) over its
-    measured kernel time (HIP events on the library's stream)."""
+    """The heaviest ERI kernel class of the last step: algorithmic FP64 flops
+    (Boys values and R table once per primitive quartet, Hermite contractions
+    per component, as counted by the library) over its measured kernel time
+    (HIP events on the library's stream)."""
     st = res["eri_stats"]
-    (lb, lk), d = max(st.items(), key=lambda kv: kv[1]["ms"])
+    key, d = max(st.items(), key=lambda kv: kv[1]["ms"])
+    name = kernel_name(key)
     sec = d["ms"] / 1e3
-    flops = d["prim_quartets"] * d["flops_per_pq"]
-    achieved = flops / sec / 1e12
+    achieved = d["flops"] / sec / 1e12
     traffic = None
     tf = ROOT / "profiles" / "eri_traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(f"k_eri_class<{lb}, {lk}>")
+            traffic = json.loads(tf.read_text()).get(name)
         except Exception:
             traffic = None
-    total_flops = sum(v["prim_quartets"] * v["flops_per_pq"] for v in st.values())
+    total_flops = sum(v["flops"] for v in st.values())
     total_ms = sum(v["ms"] for v in st.values())
-    return {"bound": "fp64-valu", "kernel": f"k_eri_class<{lb}, {lk}>", "achieved": achieved,
+    return {"bound": "fp64-valu", "kernel": name, "achieved": achieved,
             "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
             "traffic": traffic, "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
-            "flops_per_launch": flops / max(1, d["launches"]),
+            "flops_per_launch": d["flops"] / max(1, d["launches"]),
             "all_eri_classes_tflops": total_flops / (total_ms / 1e3) / 1e12}
 
 

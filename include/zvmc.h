@@ -125,14 +125,18 @@ int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
  * n[k] = number of launches of that phase.  Arrays of 8. */
 int zv_last_timings(const zv_context *ctx, double *ms, int64_t *launches);
 
-/* Per ERI class of the last zv_gauss_eri call (class index 6*LB + LK, LB >= LK
- * the Hermite orders of the two pairs, 5 = any order above 4): kernel
- * milliseconds, launches, primitive quartets evaluated and FP64 flops per
- * primitive quartet of the class kernel.  Arrays of 36.  Diagnostic entry
- * point with no counterpart in the reference (which prints only a CPU time,
- * src/two_e_slater.F90:286-287). */
+/* Per ERI kernel class of the last zv_gauss_eri / zv_gauss_ijkl_list call,
+ * arrays of 72.  Entries 0..35: component kernels, index 6*LB + LK (LB >= LK
+ * the Hermite orders of the two AO pairs, 5 = the runtime-order kernel);
+ * entries 36..71: shell kernels, index 36 + 6*LU + LL (orders of the
+ * uniform-side and lane-side shell pairs).  ms: kernel milliseconds;
+ * launches; units: primitive-quartet evaluations (times the lane-side
+ * components for shell kernels); flops: algorithmic FP64 flops (Boys values
+ * and R table once per primitive quartet, contractions per component).
+ * Diagnostic entry point with no counterpart in the reference (which prints
+ * only a CPU time, src/two_e_slater.F90:286-287). */
 int zv_last_eri_stats(const zv_context *ctx, double *ms, int64_t *launches,
-                      double *prim_quartets, double *flops_per_pq);
+                      double *units, double *flops);
 
 #ifdef __cplusplus
 }

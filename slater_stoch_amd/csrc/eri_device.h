@@ -1,0 +1,207 @@
+// Device-side pieces shared by the ERI kernels (component and shell
+// kernels): Hermite index tables, the primitive-pair layout, the Boys
+// function of src/rint_sum.F90 and the McMurchie-Davidson R table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "zvmc_internal.h"
+
+namespace zveri {
+
+constexpr double kPi52 = 34.9868366552;
+constexpr double kSqrtPi = 1.77245385090551602729816748334;
+
+constexpr int hcount(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+// Hermite index of (t,u,v): by total s = t+u+v, then t descending, u descending
+constexpr int hidx(int t, int u, int v)
+{
+    int s = t + u + v;
+    return s * (s + 1) * (s + 2) / 6 + (s - t) * (s - t + 1) / 2 + (s - t - u);
+}
+struct HDecode {
+    int t[969], u[969], v[969];
+    constexpr HDecode() : t(), u(), v()
+    {
+        for (int s = 0; s <= 16; s++)
+            for (int tt = s; tt >= 0; tt--)
+                for (int uu = s - tt; uu >= 0; uu--) {
+                    int h = hidx(tt, uu, s - tt - uu);
+                    t[h] = tt;
+                    u[h] = uu;
+                    v[h] = s - tt - uu;
+                }
+    }
+};
+constexpr HDecode kH{};
+
+// ------------------------------------------------------------------ pair data
+struct PP {
+    double *p, *ip, *Px, *Py, *Pz, *c; // [pp_total], compacted per AO pair
+    double *E;                          // [e_total]: pair P at e_off[P] + h*cap + e
+    int *cnt;                           // kept entries per AO pair
+    const int64_t *off, *eoff;          // per pair
+    const int *cap;                     // n_gauss(i) n_gauss(k)
+    const int *L;                       // Hermite order per pair
+};
+
+// ------------------------------------------------------------------ Boys
+// F_n(T) = int_0^1 t^2n exp(-T t^2) dt for n = 0..L, as src/rint_sum.F90:
+// T < 1: Taylor series of F_L (20 terms) and downward recursion;
+// T >= 1: F_0 from erf and upward recursion, exp(-T) = 0 beyond T = 80.
+template <int L> struct BoysCoef {
+    double c[20];
+    constexpr BoysCoef() : c()
+    {
+        double f = 1.0;
+        for (int k = 0; k < 20; k++) {
+            if (k > 0) f = f * k;
+            c[k] = ((k & 1) ? -1.0 : 1.0) / (f * (2 * L + 2 * k + 1));
+        }
+    }
+};
+
+template <int L> __device__ inline void boys(double T, double *F)
+{
+    if (T < 1.0) {
+        constexpr BoysCoef<L> bc{};
+        double s = bc.c[19];
+#pragma unroll
+        for (int k = 18; k >= 0; k--) s = fma(s, T, bc.c[k]);
+        F[L] = s;
+        if (L > 0) {
+            double e = exp(-T);
+#pragma unroll
+            for (int m = L - 1; m >= 0; m--) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        double u_inv = rsqrt(T);
+        double u = T * u_inv;
+        F[0] = ((0.5 * u_inv) * kSqrtPi) * erf(u);
+        if (L > 0) {
+            double e = T > 80.0 ? 0.0 : exp(-T);
+            double tri = (0.5 * u_inv) * u_inv;
+#pragma unroll
+            for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
+        }
+    }
+}
+
+// generic-L version (runtime loop)
+__device__ inline void boys_rt(int L, double T, double *F)
+{
+    if (T < 1.0) {
+        double term = 1.0, s = 1.0 / (2 * L + 1);
+        for (int k = 1; k < 40; k++) {
+            term = -term * T / k;
+            double d = term / (2 * L + 2 * k + 1);
+            s += d;
+            if (fabs(d) < 1e-18) break;
+        }
+        F[L] = s;
+        double e = exp(-T);
+        for (int m = L - 1; m >= 0; m--) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
+    } else {
+        double u_inv = rsqrt(T);
+        double u = T * u_inv;
+        F[0] = ((0.5 * u_inv) * kSqrtPi) * erf(u);
+        double e = T > 80.0 ? 0.0 : exp(-T);
+        double tri = (0.5 * u_inv) * u_inv;
+        for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
+    }
+}
+
+// ------------------------------------------------------------------ R table
+// In place, levels n = L..0, totals s descending inside a level so that the
+// level-(n+1) entries a value needs are still intact:
+//   R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{t,u,v}   (same for u, v)
+//   R^n_{000} = (-2 alpha)^n F_n
+template <int L>
+__device__ inline void hermite_R(double alpha, double X, double Y, double Z, const double *F,
+                                 double *R)
+{
+    double m2a = -2.0 * alpha;
+    double pw = 1.0;
+    double pws[L + 1];
+#pragma unroll
+    for (int n = 0; n <= L; n++) {
+        pws[n] = pw;
+        pw *= m2a;
+    }
+#pragma unroll
+    for (int n = L; n >= 0; n--) {
+#pragma unroll
+        for (int s = L - n; s >= 1; s--) {
+#pragma unroll
+            for (int t = s; t >= 0; t--) {
+#pragma unroll
+                for (int u = s - t; u >= 0; u--) {
+                    int v = s - t - u;
+                    double val;
+                    if (t > 0) {
+                        val = X * R[hidx(t - 1, u, v)];
+                        if (t > 1) val = fma((double)(t - 1), R[hidx(t - 2, u, v)], val);
+                    } else if (u > 0) {
+                        val = Y * R[hidx(t, u - 1, v)];
+                        if (u > 1) val = fma((double)(u - 1), R[hidx(t, u - 2, v)], val);
+                    } else {
+                        val = Z * R[hidx(t, u, v - 1)];
+                        if (v > 1) val = fma((double)(v - 1), R[hidx(t, u, v - 2)], val);
+                    }
+                    R[hidx(t, u, v)] = val;
+                }
+            }
+        }
+        R[0] = pws[n] * F[n];
+    }
+}
+
+__device__ inline void hermite_R_rt(int L, double alpha, double X, double Y, double Z,
+                                    const double *F, double *R)
+{
+    double m2a = -2.0 * alpha;
+    for (int n = L; n >= 0; n--) {
+        for (int s = L - n; s >= 1; s--)
+            for (int t = s; t >= 0; t--)
+                for (int u = s - t; u >= 0; u--) {
+                    int v = s - t - u;
+                    double val;
+                    if (t > 0) {
+                        val = X * R[hidx(t - 1, u, v)];
+                        if (t > 1) val += (t - 1) * R[hidx(t - 2, u, v)];
+                    } else if (u > 0) {
+                        val = Y * R[hidx(t, u - 1, v)];
+                        if (u > 1) val += (u - 1) * R[hidx(t, u - 2, v)];
+                    } else {
+                        val = Z * R[hidx(t, u, v - 1)];
+                        if (v > 1) val += (v - 1) * R[hidx(t, u, v - 2)];
+                    }
+                    R[hidx(t, u, v)] = val;
+                }
+        double pw = 1.0;
+        for (int t = 0; t < n; t++) pw *= m2a;
+        R[0] = pw * F[n];
+    }
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+} // namespace zveri
+
+// ---- host side of the shell kernels (k_eri_shell.hip) ------------------
+int zv_shell_ready(zv_context *c, const zveri::PP &pp);
+int zv_shell_run(zv_context *c, const zveri::PP &pp, const std::vector<std::pair<int, int>> &sq,
+                 DevBuf &bitem, DevBuf &bgroup, DevBuf &bpart, std::vector<char> &hplan,
+                 int64_t *ngroups);
+int zv_shell_scatter(zv_context *c, const DevBuf &bgroup, const DevBuf &bpart, int64_t ngroups,
+                     const double *pre, double *tri);
+int zv_zero(zv_context *c, double *x, int64_t n);

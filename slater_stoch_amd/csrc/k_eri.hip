@@ -33,46 +33,14 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "eri_device.h"
 #include "zvmc_internal.h"
+
+using namespace zveri;
 
 namespace {
 
-constexpr double kPi52 = 34.9868366552;
-constexpr double kSqrtPi = 1.77245385090551602729816748334;
-
-constexpr int hcount(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
-// Hermite index of (t,u,v): by total s = t+u+v, then t descending, u descending
-constexpr int hidx(int t, int u, int v)
-{
-    int s = t + u + v;
-    return s * (s + 1) * (s + 2) / 6 + (s - t) * (s - t + 1) / 2 + (s - t - u);
-}
-struct HDecode {
-    int t[969], u[969], v[969];
-    constexpr HDecode() : t(), u(), v()
-    {
-        for (int s = 0; s <= 16; s++)
-            for (int tt = s; tt >= 0; tt--)
-                for (int uu = s - tt; uu >= 0; uu--) {
-                    int h = hidx(tt, uu, s - tt - uu);
-                    t[h] = tt;
-                    u[h] = uu;
-                    v[h] = s - tt - uu;
-                }
-    }
-};
-constexpr HDecode kH{};
-
 // ------------------------------------------------------------------ pair data
-struct PP {
-    double *p, *ip, *Px, *Py, *Pz, *c; // [pp_total], compacted per AO pair
-    double *E;                          // [e_total]: pair P at e_off[P] + h*cap + e
-    int *cnt;                           // kept entries per AO pair
-    const int64_t *off, *eoff;          // per pair
-    const int *cap;                     // n_gauss(i) n_gauss(k)
-    const int *L;                       // Hermite order per pair
-};
-
 struct BasisE {
     const double *center, *c1, *c2, *g;
     const int *npower, *itype, *ngauss;
@@ -191,187 +159,22 @@ __global__ __launch_bounds__(64) void k_primpairs(BasisE B, const int *pair_i, c
     if (lane == 0) pp.cnt[P] = base;
 }
 
-// ------------------------------------------------------------------ Boys
-// F_n(T) = int_0^1 t^2n exp(-T t^2) dt for n = 0..L, as src/rint_sum.F90:
-// T < 1: Taylor series of F_L (20 terms) and downward recursion;
-// T >= 1: F_0 from erf and upward recursion, exp(-T) = 0 beyond T = 80.
-template <int L> struct BoysCoef {
-    double c[20];
-    constexpr BoysCoef() : c()
-    {
-        double f = 1.0;
-        for (int k = 0; k < 20; k++) {
-            if (k > 0) f = f * k;
-            c[k] = ((k & 1) ? -1.0 : 1.0) / (f * (2 * L + 2 * k + 1));
-        }
-    }
-};
-
-template <int L> __device__ inline void boys(double T, double *F)
-{
-    if (T < 1.0) {
-        constexpr BoysCoef<L> bc{};
-        double s = bc.c[19];
-#pragma unroll
-        for (int k = 18; k >= 0; k--) s = fma(s, T, bc.c[k]);
-        F[L] = s;
-        if (L > 0) {
-            double e = exp(-T);
-#pragma unroll
-            for (int m = L - 1; m >= 0; m--) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
-        }
-    } else {
-        double u_inv = rsqrt(T);
-        double u = T * u_inv;
-        F[0] = ((0.5 * u_inv) * kSqrtPi) * erf(u);
-        if (L > 0) {
-            double e = T > 80.0 ? 0.0 : exp(-T);
-            double tri = (0.5 * u_inv) * u_inv;
-#pragma unroll
-            for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
-        }
-    }
-}
-
-// generic-L version (runtime loop)
-__device__ inline void boys_rt(int L, double T, double *F)
-{
-    if (T < 1.0) {
-        double term = 1.0, s = 1.0 / (2 * L + 1);
-        for (int k = 1; k < 40; k++) {
-            term = -term * T / k;
-            double d = term / (2 * L + 2 * k + 1);
-            s += d;
-            if (fabs(d) < 1e-18) break;
-        }
-        F[L] = s;
-        double e = exp(-T);
-        for (int m = L - 1; m >= 0; m--) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
-    } else {
-        double u_inv = rsqrt(T);
-        double u = T * u_inv;
-        F[0] = ((0.5 * u_inv) * kSqrtPi) * erf(u);
-        double e = T > 80.0 ? 0.0 : exp(-T);
-        double tri = (0.5 * u_inv) * u_inv;
-        for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
-    }
-}
-
-// ------------------------------------------------------------------ R table
-// In place, levels n = L..0, totals s descending inside a level so that the
-// level-(n+1) entries a value needs are still intact:
-//   R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{t,u,v}   (same for u, v)
-//   R^n_{000} = (-2 alpha)^n F_n
-template <int L>
-__device__ inline void hermite_R(double alpha, double X, double Y, double Z, const double *F,
-                                 double *R)
-{
-    double m2a = -2.0 * alpha;
-    double pw = 1.0;
-    double pws[L + 1];
-#pragma unroll
-    for (int n = 0; n <= L; n++) {
-        pws[n] = pw;
-        pw *= m2a;
-    }
-#pragma unroll
-    for (int n = L; n >= 0; n--) {
-#pragma unroll
-        for (int s = L - n; s >= 1; s--) {
-#pragma unroll
-            for (int t = s; t >= 0; t--) {
-#pragma unroll
-                for (int u = s - t; u >= 0; u--) {
-                    int v = s - t - u;
-                    double val;
-                    if (t > 0) {
-                        val = X * R[hidx(t - 1, u, v)];
-                        if (t > 1) val = fma((double)(t - 1), R[hidx(t - 2, u, v)], val);
-                    } else if (u > 0) {
-                        val = Y * R[hidx(t, u - 1, v)];
-                        if (u > 1) val = fma((double)(u - 1), R[hidx(t, u - 2, v)], val);
-                    } else {
-                        val = Z * R[hidx(t, u, v - 1)];
-                        if (v > 1) val = fma((double)(v - 1), R[hidx(t, u, v - 2)], val);
-                    }
-                    R[hidx(t, u, v)] = val;
-                }
-            }
-        }
-        R[0] = pws[n] * F[n];
-    }
-}
-
-__device__ inline void hermite_R_rt(int L, double alpha, double X, double Y, double Z,
-                                    const double *F, double *R)
-{
-    double m2a = -2.0 * alpha;
-    for (int n = L; n >= 0; n--) {
-        for (int s = L - n; s >= 1; s--)
-            for (int t = s; t >= 0; t--)
-                for (int u = s - t; u >= 0; u--) {
-                    int v = s - t - u;
-                    double val;
-                    if (t > 0) {
-                        val = X * R[hidx(t - 1, u, v)];
-                        if (t > 1) val += (t - 1) * R[hidx(t - 2, u, v)];
-                    } else if (u > 0) {
-                        val = Y * R[hidx(t, u - 1, v)];
-                        if (u > 1) val += (u - 1) * R[hidx(t, u - 2, v)];
-                    } else {
-                        val = Z * R[hidx(t, u, v - 1)];
-                        if (v > 1) val += (v - 1) * R[hidx(t, u, v - 2)];
-                    }
-                    R[hidx(t, u, v)] = val;
-                }
-        double pw = 1.0;
-        for (int t = 0; t < n; t++) pw *= m2a;
-        R[0] = pw * F[n];
-    }
-}
-
 // ------------------------------------------------------------------ classes
-enum QMode { Q_DIAG = 0, Q_TRI = 1, Q_RECT = 2, Q_LIST = 3 };
+// Explicit quartet lists of AO pairs (bra, ket).  Q_LIST writes quartet x to
+// out[x]; Q_PAIRS writes it to its triangle slot out[tri(max, min)].
+enum QMode { Q_LIST = 0, Q_PAIRS = 1 };
 
 struct QSrc {
     int mode;
-    const int *listB, *listK; // pair lists of the two classes
-    int nB, nK;
-    const int *qb, *qk;       // explicit bra / ket pairs (Q_LIST)
+    const int *qb, *qk;
     int64_t nq;
-    const double *pre;        // Schwarz diagonal per pair (Q_TRI, Q_RECT), may be null
 };
 
 __device__ inline void quartet_of(const QSrc &S, int64_t x, int &Pb, int &Pk, int64_t &oidx)
 {
-    if (S.mode == Q_DIAG) {
-        Pb = Pk = S.listB[x];
-        oidx = zv_tri(Pb, Pb);
-    } else if (S.mode == Q_TRI) {
-        // strict lower triangle of the list: x -> (xi > yi)
-        int64_t xi = (int64_t)((sqrt(8.0 * (double)x + 1.0) + 1.0) * 0.5);
-        while (xi * (xi - 1) / 2 > x) xi--;
-        while ((xi + 1) * xi / 2 <= x) xi++;
-        int64_t yi = x - xi * (xi - 1) / 2;
-        Pb = S.listB[xi];
-        Pk = S.listB[yi];
-        oidx = Pb > Pk ? zv_tri(Pb, Pk) : zv_tri(Pk, Pb);
-    } else if (S.mode == Q_RECT) {
-        Pb = S.listB[x / S.nK];
-        Pk = S.listK[x % S.nK];
-        oidx = Pb > Pk ? zv_tri(Pb, Pk) : zv_tri(Pk, Pb);
-    } else {
-        Pb = S.qb[x];
-        Pk = S.qk[x];
-        oidx = x;
-    }
-}
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    Pb = S.qb[x];
+    Pk = S.qk[x];
+    oidx = S.mode == Q_LIST ? x : (Pb > Pk ? zv_tri(Pb, Pk) : zv_tri(Pk, Pb));
 }
 
 // Loop order: each lane owns one ket primitive pair (its position, exponent,
@@ -384,7 +187,9 @@ __global__ __launch_bounds__(256) void k_eri_class(PP pp, QSrc S, int nsplit, do
 {
     constexpr int HB = hcount(LB), HK = hcount(LK), L = LB + LK, HR = hcount(L);
     int lane = threadIdx.x & 63;
-    int64_t item = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave index made provably uniform so the per-quartet data come in by scalar loads
+    int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                   __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (item >= S.nq * nsplit) return;
     int64_t x = item / nsplit;
     int sp = (int)(item % nsplit);
@@ -392,9 +197,7 @@ __global__ __launch_bounds__(256) void k_eri_class(PP pp, QSrc S, int nsplit, do
     int64_t oidx;
     quartet_of(S, x, Pb, Pk, oidx);
     double acc = 0.0;
-    bool skip = S.pre && (S.mode == Q_TRI || S.mode == Q_RECT) &&
-                fabs(S.pre[Pb] * S.pre[Pk]) < 1.0e-30;
-    if (!skip) {
+    {
         int nb = pp.cnt[Pb], nk = pp.cnt[Pk];
         int chunk = (nb + nsplit - 1) / nsplit;
         int a0 = sp * chunk, a1 = min(nb, a0 + chunk);
@@ -452,7 +255,9 @@ __global__ __launch_bounds__(64) void k_eri_generic(PP pp, QSrc S, int nsplit, d
                                                     double *part)
 {
     int lane = threadIdx.x & 63;
-    int64_t item = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave index made provably uniform so the per-quartet data come in by scalar loads
+    int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                   __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (item >= S.nq * nsplit) return;
     int64_t x = item / nsplit;
     int sp = (int)(item % nsplit);
@@ -460,9 +265,7 @@ __global__ __launch_bounds__(64) void k_eri_generic(PP pp, QSrc S, int nsplit, d
     int64_t oidx;
     quartet_of(S, x, Pb, Pk, oidx);
     double acc = 0.0;
-    bool skip = S.pre && (S.mode == Q_TRI || S.mode == Q_RECT) &&
-                fabs(S.pre[Pb] * S.pre[Pk]) < 1.0e-30;
-    if (!skip) {
+    {
         int LB = pp.L[Pb], LK = pp.L[Pk], L = LB + LK;
         int HB = hcount(LB), HK = hcount(LK);
         int nb = pp.cnt[Pb], nk = pp.cnt[Pk];
@@ -617,32 +420,23 @@ static int ensure_primpairs(zv_context *c, PP &pp)
     return ZV_OK;
 }
 
-// primitive quartets a class launch evaluates (screened quartets excluded)
-double class_pq(const zv_context *c, int mode, const std::vector<int> &lb,
-                       const std::vector<int> &lk, const std::vector<double> *pre)
+// FP64 flops per primitive quartet of the (LB, LK) component kernel: Boys
+// values (3 per order plus ~25 for erf/exp/sqrt/division), the Hermite R
+// recursion (2 FMA per entry over all levels), the ket-Hermite contraction
+// (one FMA per pair of Hermite indices), the bra sum and the prefactor.
+static double component_flops(int LB, int LK)
 {
-    const std::vector<int> &cnt = c->h_cnt;
-    auto keep = [&](int P, int Q) {
-        return !pre || fabs((*pre)[P] * (*pre)[Q]) >= 1.0e-30;
-    };
-    double s = 0.0;
-    if (mode == Q_DIAG) {
-        for (int P : lb) s += (double)cnt[P] * cnt[P];
-    } else if (mode == Q_TRI) {
-        for (size_t x = 0; x < lb.size(); x++)
-            for (size_t y = 0; y < x; y++)
-                if (keep(lb[x], lb[y])) s += (double)cnt[lb[x]] * cnt[lb[y]];
-    } else {
-        for (int P : lb)
-            for (int Q : lk)
-                if (keep(P, Q)) s += (double)cnt[P] * cnt[Q];
-    }
-    return s;
+    int L = LB + LK;
+    double HB = hcount(LB), HK = hcount(LK);
+    double HR4 = (L + 1) * (L + 2) * (L + 3) * (L + 4) / 24.0; // R entries over all levels
+    return 25.0 + 3.0 * (L + 1) + 4.0 * (HR4 - 1.0) + 2.0 * HB * HK + 2.0 * HB + 12.0;
 }
 
-// Launch one class over the quartet source S, splitting bra primitive pairs
-// when there are too few quartets to fill the device.
-static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, double *out)
+// Launch one class over an explicit quartet list S, splitting bra primitive
+// pairs when there are too few quartets to fill the device.  pq and flops
+// (primitive quartets and algorithmic flops of the list) feed the statistics.
+static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, double *out,
+                        double pq, double flops)
 {
     if (S.nq <= 0) return ZV_OK;
     const int64_t target_waves = 16384;
@@ -654,14 +448,10 @@ static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, dou
         part = (double *)c->b_part.p;
     }
     int64_t items = S.nq * nsplit;
-    static const bool trace = getenv("ZV_TRACE") != nullptr;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (trace) {
-        hipEventCreate(&t0);
-        hipEventCreate(&t1);
-        hipEventRecord(t0, c->stream);
-    }
-    PhaseTimer tm(c, PH_ERI, LB * 6 + LK);
+    const int cls = LB * 6 + LK;
+    c->cls_pq[cls] += pq;
+    c->cls_flops[cls] += flops;
+    PhaseTimer tm(c, PH_ERI, cls);
     if (LB <= ZV_LMAX_T && LK <= ZV_LMAX_T) {
         if (!launch_class_t(LB, LK, nblk(items * 64, 256), c->stream, pp, S, nsplit, out, part))
             return zv_fail(c, ZV_ERR_STATE, "no ERI kernel for class (%d,%d)", LB, LK);
@@ -675,83 +465,121 @@ static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, dou
                            nsplit, (const double *)part, out);
         ZV_HIPCHECK(c, hipGetLastError());
     }
-    if (trace) {
-        hipEventRecord(t1, c->stream);
-        hipEventSynchronize(t1);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, t0, t1);
-        fprintf(stderr, "[zvmc] eri class (%d,%d) mode %d quartets %lld split %d: %.3f ms\n", LB,
-                LK, S.mode, (long long)S.nq, nsplit, ms);
-        hipEventDestroy(t0);
-        hipEventDestroy(t1);
-    }
     return ZV_OK;
 }
 
-// pairs grouped by Hermite order (classes of the generic kernel are merged:
-// every pair with L > ZV_LMAX_T goes through k_eri_generic, which reads L
-// per pair, so they form one list with a pseudo-order of ZV_LMAX_T + 1)
-static void class_lists(zv_context *c, std::vector<std::vector<int>> &lists)
+// Component quartets (bra, ket) of pairs beyond the templated orders, written
+// to their triangle slots by the runtime-order kernel.
+static int run_generic_pairs(zv_context *c, const PP &pp, const std::vector<int> &qb,
+                             const std::vector<int> &qk, double *tri)
 {
-    lists.assign(ZV_LMAX_T + 2, {});
-    for (int P = 0; P < c->npair; P++) {
-        int L = c->h_pair_L[P];
-        lists[L > ZV_LMAX_T ? ZV_LMAX_T + 1 : L].push_back(P);
+    const int64_t nq = (int64_t)qb.size();
+    if (!nq) return ZV_OK;
+    double pq = 0.0, flops = 0.0;
+    for (int64_t t = 0; t < nq; t++) {
+        double v = (double)c->h_cnt[qb[t]] * c->h_cnt[qk[t]];
+        pq += v;
+        flops += v * component_flops(c->h_pair_L[qb[t]], c->h_pair_L[qk[t]]);
     }
+    c->h_plist.resize(2 * nq);
+    std::copy(qb.begin(), qb.end(), c->h_plist.begin());
+    std::copy(qk.begin(), qk.end(), c->h_plist.begin() + nq);
+    ZV_CHECK(zv_ensure(c, c->b_plist, sizeof(int) * 2 * nq));
+    int *d = (int *)c->b_plist.p;
+    ZV_HIPCHECK(c, hipMemcpyAsync(d, c->h_plist.data(), sizeof(int) * 2 * nq,
+                                  hipMemcpyHostToDevice, c->stream));
+    QSrc S{Q_PAIRS, d, d + nq, nq};
+    return launch_class(c, pp, S, ZV_LMAX_T + 1, ZV_LMAX_T + 1, tri, pq, flops);
 }
 
+// The reference's Schwarz driver (src/two_e_slater.F90:230-273) over shell
+// quartets.  Phase 1: the diagonal shell quartets (x|x), which hold every
+// (P|P) = precond(i,k).  Phase 2: the shell quartets (x|y), x > y, that hold
+// at least one component quartet passing |precond_P precond_Q| >= 1e-30;
+// the test itself is applied per component quartet when the results are
+// written.  Shell pairs of order above ZV_LMAX_T go through explicit
+// component lists.
 int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_screened)
 {
     PP pp;
     ZV_CHECK(ensure_primpairs(c, pp));
-    int np = c->npair;
-    std::vector<std::vector<int>> lists;
-    class_lists(c, lists);
-    int ncls = (int)lists.size();
-    std::vector<int> flat;
-    std::vector<int64_t> lo(ncls + 1, 0);
-    for (int t = 0; t < ncls; t++) {
-        lo[t] = (int64_t)flat.size();
-        flat.insert(flat.end(), lists[t].begin(), lists[t].end());
-    }
-    lo[ncls] = (int64_t)flat.size();
-    ZV_CHECK(zv_ensure(c, c->b_list, sizeof(int) * flat.size()));
-    ZV_HIPCHECK(c, hipMemcpyAsync(c->b_list.p, flat.data(), sizeof(int) * flat.size(),
-                                  hipMemcpyHostToDevice, c->stream));
-    const int *dl = (const int *)c->b_list.p;
+    ZV_CHECK(zv_shell_ready(c, pp));
+    const int np = c->npair, nsp = c->nsp;
     ZV_CHECK(zv_ensure(c, c->b_tri, sizeof(double) * c->nint));
     ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * np));
     double *tri = (double *)c->b_tri.p, *pre = (double *)c->b_misc.p;
-    auto order_of = [&](int t) { return t > ZV_LMAX_T ? ZV_LMAX_T + 1 : t; };
-    // phase 1: diagonal quartets (ik|ik) = precond
-    for (int t = 0; t < ncls; t++) {
-        QSrc S{Q_DIAG, dl + lo[t], dl + lo[t], (int)lists[t].size(), (int)lists[t].size(),
-               nullptr, nullptr, (int64_t)lists[t].size(), nullptr};
-        ZV_CHECK(launch_class(c, pp, S, order_of(t), order_of(t), tri));
+    ZV_CHECK(zv_zero(c, tri, c->nint));
+    auto templ = [&](int s) { return c->h_sp_L[s] <= ZV_LMAX_T; };
+    auto comps = [&](int s) {
+        return std::make_pair(c->h_sp_comp.data() + c->h_sp_coff[s],
+                              c->h_sp_comp.data() + c->h_sp_coff[s + 1]);
+    };
+    // phase 1
+    std::vector<std::pair<int, int>> sq;
+    std::vector<int> gb, gk;
+    for (int s = 0; s < nsp; s++) {
+        if (!c->h_sp_np[s]) continue;
+        if (templ(s)) {
+            sq.push_back({s, s});
+        } else {
+            auto r = comps(s);
+            for (const int *P = r.first; P != r.second; P++) {
+                gb.push_back(*P);
+                gk.push_back(*P);
+            }
+        }
     }
+    int64_t ng1 = 0, ng2 = 0;
+    ZV_CHECK(zv_shell_run(c, pp, sq, c->b_sitem1, c->b_sgroup1, c->b_part1, c->h_plan1, &ng1));
+    ZV_CHECK(zv_shell_scatter(c, c->b_sgroup1, c->b_part1, ng1, nullptr, tri));
+    ZV_CHECK(run_generic_pairs(c, pp, gb, gk, tri));
     hipLaunchKernelGGL(k_diag_pre, dim3(nblk(np, 256)), dim3(256), 0, c->stream, np, tri, pre);
     ZV_HIPCHECK(c, hipGetLastError());
     std::vector<double> hpre(np);
     ZV_HIPCHECK(c, hipMemcpyAsync(hpre.data(), pre, sizeof(double) * np, hipMemcpyDeviceToHost,
                                   c->stream));
-    // phase 2: off-diagonal quartets with the Cauchy-Schwarz test
-    for (int tb = 0; tb < ncls; tb++)
-        for (int tk = 0; tk <= tb; tk++) {
-            int nb = (int)lists[tb].size(), nk = (int)lists[tk].size();
-            QSrc S;
-            if (tb == tk)
-                S = QSrc{Q_TRI, dl + lo[tb], dl + lo[tb], nb, nb, nullptr, nullptr,
-                         (int64_t)nb * (nb - 1) / 2, pre};
-            else
-                S = QSrc{Q_RECT, dl + lo[tb], dl + lo[tk], nb, nk, nullptr, nullptr,
-                         (int64_t)nb * nk, pre};
-            ZV_CHECK(launch_class(c, pp, S, order_of(tb), order_of(tk), tri));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    // phase 2
+    std::vector<double> spmax(nsp, 0.0);
+    for (int s = 0; s < nsp; s++) {
+        auto r = comps(s);
+        for (const int *P = r.first; P != r.second; P++)
+            spmax[s] = std::max(spmax[s], fabs(hpre[*P]));
+    }
+    sq.clear();
+    gb.clear();
+    gk.clear();
+    for (int x = 0; x < nsp; x++) {
+        if (!c->h_sp_np[x]) continue;
+        auto rx = comps(x);
+        if (!templ(x))
+            for (const int *P = rx.first; P != rx.second; P++)
+                for (const int *Q = rx.first; Q != rx.second; Q++)
+                    if (*P > *Q && fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
+                        gb.push_back(*P);
+                        gk.push_back(*Q);
+                    }
+        for (int y = 0; y < x; y++) {
+            if (!c->h_sp_np[y] || spmax[x] * spmax[y] < 1.0e-30) continue;
+            if (templ(x) && templ(y)) {
+                sq.push_back({x, y});
+                continue;
+            }
+            auto ry = comps(y);
+            for (const int *P = rx.first; P != rx.second; P++)
+                for (const int *Q = ry.first; Q != ry.second; Q++)
+                    if (fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
+                        gb.push_back(*P);
+                        gk.push_back(*Q);
+                    }
         }
+    }
+    ZV_CHECK(zv_shell_run(c, pp, sq, c->b_sitem2, c->b_sgroup2, c->b_part2, c->h_plan2, &ng2));
+    ZV_CHECK(zv_shell_scatter(c, c->b_sgroup2, c->b_part2, ng2, pre, tri));
+    // the off-diagonal components of (x|x) get the test now that precond is known
+    ZV_CHECK(zv_shell_scatter(c, c->b_sgroup1, c->b_part1, ng1, pre, tri));
+    ZV_CHECK(run_generic_pairs(c, pp, gb, gk, tri));
     ZV_CHECK(zv_run_scatter_ref(c, tri, ijkl));
-    // work accounting is done on request (zv_last_eri_stats)
-    c->last_lists = lists;
-    c->last_pre = hpre;
-    c->pq_pending = true;
     if (precond) {
         int n = c->n;
         for (int i = 1; i <= n; i++)
@@ -804,12 +632,16 @@ int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, c
     std::vector<int> qb, qk2;
     std::vector<int64_t> perm;
     std::vector<int64_t> lo(ncls * ncls + 1, 0);
+    std::vector<double> bpq(ncls * ncls, 0.0), bfl(ncls * ncls, 0.0);
     for (int b = 0; b < ncls * ncls; b++) {
         lo[b] = (int64_t)perm.size();
         for (int64_t t : bucket[b]) {
             perm.push_back(t);
             qb.push_back(pb[t]);
             qk2.push_back(pk[t]);
+            double v = (double)c->h_cnt[pb[t]] * c->h_cnt[pk[t]];
+            bpq[b] += v;
+            bfl[b] += v * component_flops(c->h_pair_L[pb[t]], c->h_pair_L[pk[t]]);
         }
     }
     lo[ncls * ncls] = (int64_t)perm.size();
@@ -824,8 +656,8 @@ int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, c
     for (int b = 0; b < ncls * ncls; b++) {
         int64_t cnt = lo[b + 1] - lo[b];
         if (!cnt) continue;
-        QSrc S{Q_LIST, nullptr, nullptr, 0, 0, dqb + lo[b], dqk + lo[b], cnt, nullptr};
-        ZV_CHECK(launch_class(c, pp, S, b / ncls, b % ncls, dout + lo[b]));
+        QSrc S{Q_LIST, dqb + lo[b], dqk + lo[b], cnt};
+        ZV_CHECK(launch_class(c, pp, S, b / ncls, b % ncls, dout + lo[b], bpq[b], bfl[b]));
     }
     std::vector<double> h(nq);
     ZV_HIPCHECK(c, hipMemcpyAsync(h.data(), dout, sizeof(double) * nq, hipMemcpyDeviceToHost,
@@ -833,21 +665,4 @@ int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, c
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     for (int64_t t = 0; t < nq; t++) out[perm[t]] = h[t];
     return ZV_OK;
-}
-
-// per-class primitive-quartet counts of the last zv_gauss_eri (host work)
-void zv_eri_account(const zv_context *cc)
-{
-    zv_context *c = const_cast<zv_context *>(cc);
-    if (!c->pq_pending) return;
-    const auto &lists = c->last_lists;
-    int ncls = (int)lists.size();
-    auto order_of = [&](int t) { return t > ZV_LMAX_T ? ZV_LMAX_T + 1 : t; };
-    for (int t = 0; t < ncls; t++)
-        c->cls_pq[order_of(t) * 6 + order_of(t)] += class_pq(c, Q_DIAG, lists[t], lists[t], nullptr);
-    for (int tb = 0; tb < ncls; tb++)
-        for (int tk = 0; tk <= tb; tk++)
-            c->cls_pq[order_of(tb) * 6 + order_of(tk)] +=
-                class_pq(c, tb == tk ? Q_TRI : Q_RECT, lists[tb], lists[tk], &c->last_pre);
-    c->pq_pending = false;
 }

@@ -75,12 +75,12 @@ void zv_reset_timings(zv_context *c)
         c->t_ms[p] = 0.0;
         c->t_n[p] = 0;
     }
-    for (int k = 0; k < 36; k++) {
+    for (int k = 0; k < 72; k++) {
         c->cls_ms[k] = 0.0;
         c->cls_n[k] = 0;
         c->cls_pq[k] = 0.0;
+        c->cls_flops[k] = 0.0;
     }
-    c->pq_pending = false;
     c->pend_phase.clear();
     c->pend_cls.clear();
     c->pend_ev.clear();
@@ -96,7 +96,7 @@ int zv_collect_timings(zv_context *c)
             c->t_ms[c->pend_phase[t]] += ms;
             c->t_n[c->pend_phase[t]] += 1;
             int k = c->pend_cls[t];
-            if (k >= 0 && k < 36) {
+            if (k >= 0 && k < 72) {
                 c->cls_ms[k] += ms;
                 c->cls_n[k] += 1;
             }
@@ -194,6 +194,7 @@ static void free_basis(zv_context *c)
     c->d_pair_f = c->d_pair_jf = c->d_pair_G = nullptr;
     c->d_kbase = nullptr;
     c->pp_ready = false;
+    c->sp_ready = false;
 }
 
 extern "C" int zv_destroy(zv_context *c)
@@ -205,7 +206,9 @@ extern "C" int zv_destroy(zv_context *c)
     zv_blas_destroy(c);
     DevBuf *bufs[] = {&c->b_r1, &c->b_r2, &c->b_pi0, &c->b_pd, &c->b_accS, &c->b_accG,
                       &c->b_s1, &c->b_s2, &c->b_out, &c->b_out2, &c->b_blk, &c->b_pp,
-                      &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc};
+                      &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc,
+                      &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
+                      &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist};
     for (DevBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -232,29 +235,15 @@ extern "C" int zv_last_timings(const zv_context *c, double *ms, int64_t *launche
     return ZV_OK;
 }
 
-// FP64 flops per primitive quartet of the (LB, LK) class kernel: Boys values
-// (3 per order plus ~25 for erf/exp/sqrt/division), the Hermite R recursion
-// (2 FMA per entry), the ket-Hermite contraction (one FMA per pair of
-// Hermite indices), the bra sum (one FMA per bra index) and the prefactor.
-static double eri_flops_per_pq(int LB, int LK)
-{
-    int L = LB + LK;
-    double HB = (LB + 1) * (LB + 2) * (LB + 3) / 6.0, HK = (LK + 1) * (LK + 2) * (LK + 3) / 6.0;
-    double HR4 = (L + 1) * (L + 2) * (L + 3) * (L + 4) / 24.0; // R entries over all levels
-    return 25.0 + 3.0 * (L + 1) + 4.0 * (HR4 - 1.0) + 2.0 * HB * HK + 2.0 * HB + 12.0;
-}
-
 extern "C" int zv_last_eri_stats(const zv_context *c, double *ms, int64_t *launches,
-                                 double *prim_quartets, double *flops_per_pq)
+                                 double *units, double *flops)
 {
     if (!c) return ZV_ERR_ARG;
-    zv_eri_account(c);
-    for (int k = 0; k < 36; k++) {
-        int LB = k / 6, LK = k % 6;
+    for (int k = 0; k < 72; k++) {
         if (ms) ms[k] = c->cls_ms[k];
         if (launches) launches[k] = c->cls_n[k];
-        if (prim_quartets) prim_quartets[k] = c->cls_pq[k];
-        if (flops_per_pq) flops_per_pq[k] = (LB < 5 && LK < 5) ? eri_flops_per_pq(LB, LK) : 0.0;
+        if (units) units[k] = c->cls_pq[k];
+        if (flops) flops[k] = c->cls_flops[k];
     }
     return ZV_OK;
 }
@@ -323,6 +312,39 @@ extern "C" int zv_set_basis(zv_context *c, int n, const double *center, const in
         c->h_lao[i] = npower[3 * i] + npower[3 * i + 1] + npower[3 * i + 2] +
                       (i_type[i] == 2 ? 2 : 0);
     }
+    // shells: consecutive AOs with identical center, fit and order
+    auto same_fit = [&](int a, int b) {
+        if (i_type[a] != i_type[b] || n_gauss[a] != n_gauss[b] || c->h_lao[a] != c->h_lao[b])
+            return false;
+        for (int d = 0; d < 3; d++)
+            if (center[3 * a + d] != center[3 * b + d]) return false;
+        for (int m = 0; m < n_gauss[a]; m++) {
+            size_t ia = (size_t)a * ZV_GMAX + m, ib = (size_t)b * ZV_GMAX + m;
+            if (c->h_g[ia] != c->h_g[ib] || c->h_c1[ia] != c->h_c1[ib] || c->h_c2[ia] != c->h_c2[ib])
+                return false;
+        }
+        return true;
+    };
+    c->h_sh_first.clear();
+    for (int i = 0; i < n; i++)
+        if (i == 0 || i - c->h_sh_first.back() >= ZV_SHMAX || !same_fit(i - 1, i))
+            c->h_sh_first.push_back(i);
+    c->nshell = (int)c->h_sh_first.size();
+    c->h_sh_first.push_back(n);
+    c->nsp = c->nshell * (c->nshell + 1) / 2;
+    c->h_sp_coff.assign(c->nsp + 1, 0);
+    c->h_sp_L.assign(c->nsp, 0);
+    c->h_sp_comp.clear();
+    for (int I = 0; I < c->nshell; I++)
+        for (int K = 0; K <= I; K++) {
+            int sp = I * (I + 1) / 2 + K;
+            c->h_sp_coff[sp] = (int)c->h_sp_comp.size();
+            c->h_sp_L[sp] = c->h_lao[c->h_sh_first[I]] + c->h_lao[c->h_sh_first[K]];
+            for (int i = c->h_sh_first[I]; i < c->h_sh_first[I + 1]; i++)
+                for (int k = c->h_sh_first[K]; k < c->h_sh_first[K + 1] && k <= i; k++)
+                    c->h_sp_comp.push_back(zv_pidx(i + 1, k + 1));
+        }
+    c->h_sp_coff[c->nsp] = (int)c->h_sp_comp.size();
     // pair table, lexicographic (i, k), i >= k
     int np = c->npair, npp = c->npair_pad;
     c->h_pair_i.assign(np, 0);

@@ -17,6 +17,8 @@
 #define ZV_SUB 32      // samples per LDS stage of the fast estimator
 #define ZV_LMAX_T 4    // largest pair angular momentum with a templated ERI kernel
 #define ZV_LPAIR_MAX 8 // largest pair angular momentum supported at all
+#define ZV_SHMAX 15    // largest shell (Cartesian components sharing one Gaussian fit)
+#define ZV_MAXU 36     // components of the uniform side handled by one shell work item
 
 struct DevBuf {
     void *p = nullptr;
@@ -43,6 +45,13 @@ struct zv_context {
     std::vector<int64_t> h_pp_off;         // primitive-pair slot offset of pair p
     std::vector<int64_t> h_e_off;          // Hermite-coefficient offset of pair p
     int64_t pp_total = 0, e_total = 0;
+    // shells: runs of consecutive AOs with the same center, Gaussian fit and
+    // order; their pairs share every primitive-pair quantity but E
+    int nshell = 0, nsp = 0;
+    std::vector<int> h_sh_first;           // first AO (0-based) of each shell, nshell + 1
+    std::vector<int> h_sp_coff;            // component offset of shell pair (I >= K), nsp + 1
+    std::vector<int> h_sp_comp;            // AO pair index of each component
+    std::vector<int> h_sp_L;               // Hermite order of the shell pair
 
     // basis (device)
     double *d_center = nullptr, *d_zeta = nullptr, *d_c1 = nullptr, *d_c2 = nullptr,
@@ -67,6 +76,15 @@ struct zv_context {
     DevBuf b_part;                     // split partial sums
     DevBuf b_misc;
     bool pp_ready = false;
+    // shell-pair data: component tables and the uniform-side copies
+    DevBuf b_spmeta;                   // comp[npair], coff[nsp+1], puoff[nsp], euoff[nsp]
+    DevBuf b_spu, b_spe;               // per primitive pair: p,1/p,P,c; E [a][comp][h]
+    std::vector<int> h_sp_np;          // kept primitive pairs per shell pair
+    bool sp_ready = false;
+    // planned shell work of the two Schwarz phases (host copies kept alive for the copies)
+    DevBuf b_sitem1, b_sgroup1, b_part1, b_sitem2, b_sgroup2, b_part2, b_plist;
+    std::vector<char> h_plan1, h_plan2;
+    std::vector<int> h_plist;
 
     // timings of the last call: events recorded around each launch group
     std::vector<hipEvent_t> ev_pool;
@@ -76,17 +94,15 @@ struct zv_context {
     double t_ms[PH_N] = {};
     int64_t t_n[PH_N] = {};
     std::vector<int> pend_cls;        // ERI class of each pending timer (-1: none)
-    double cls_ms[36] = {};
-    int64_t cls_n[36] = {};
-    double cls_pq[36] = {};           // primitive quartets evaluated per class
+    // ERI statistics: [0, 36) component kernels 6 LB + LK, [36, 72) shell kernels
+    double cls_ms[72] = {};
+    int64_t cls_n[72] = {};
+    double cls_pq[72] = {};           // primitive-quartet evaluations per class
+    double cls_flops[72] = {};        // algorithmic FP64 flops per class
     std::vector<int> h_cnt;           // kept primitive pairs per AO pair (host copy)
-    std::vector<std::vector<int>> last_lists; // pair lists by Hermite order, last run
-    std::vector<double> last_pre;     // Schwarz diagonal of the last run
-    bool pq_pending = false;
     void *blas = nullptr;             // rocBLAS handle (cleanup GEMMs)
 };
 void zv_blas_destroy(zv_context *c);
-void zv_eri_account(const zv_context *c);
 
 // ---- helpers (zvmc_api.hip) -------------------------------------------
 int zv_fail(zv_context *c, int code, const char *fmt, ...);

@@ -202,16 +202,19 @@ class Engine:
         return e, int(nrem[0])
 
     def eri_stats(self):
-        """Per (LB, LK) class of the last gauss_eri: ms, launches, primitive
-        quartets, FP64 flops per primitive quartet."""
-        ms, pq, fl = np.zeros(36), np.zeros(36), np.zeros(36)
-        n = np.zeros(36, np.int64)
-        self._check(self.lib.zv_last_eri_stats(self.h, _p(ms), _p(n), _p(pq), _p(fl)))
+        """Per kernel class of the last gauss_eri / gauss_ijkl: {(kind, LB, LK):
+        dict(ms, launches, units, flops)} with kind "component" (orders of the
+        two AO pairs) or "shell" (orders of the uniform and lane shell pairs)."""
+        ms, un, fl = np.zeros(72), np.zeros(72), np.zeros(72)
+        n = np.zeros(72, np.int64)
+        self._check(self.lib.zv_last_eri_stats(self.h, _p(ms), _p(n), _p(un), _p(fl)))
         out = {}
-        for k in range(36):
-            if n[k] or pq[k]:
-                out[(k // 6, k % 6)] = dict(ms=float(ms[k]), launches=int(n[k]),
-                                            prim_quartets=float(pq[k]), flops_per_pq=float(fl[k]))
+        for k in range(72):
+            if n[k] or un[k]:
+                kind = "shell" if k >= 36 else "component"
+                j = k - 36 if k >= 36 else k
+                out[(kind, j // 6, j % 6)] = dict(ms=float(ms[k]), launches=int(n[k]),
+                                                  units=float(un[k]), flops=float(fl[k]))
         return out
 
     def timings(self):

@@ -217,6 +217,53 @@ def test_gauss_eri_full_list_equals_explicit_list(engine):
     np.testing.assert_allclose(c, a, rtol=1e-12, atol=1e-16)
 
 
+def _kcp_lookup(is_, js, ks, ls, n):
+    """Reference position of quartet (i k | j l) (any of its 8 equivalent forms)."""
+    key = lambda a, b, c, d: ((a.astype(np.int64) * (n + 1) + b) * (n + 1) + c) * (n + 1) + d
+    keys = key(is_, js, ks, ls)
+    order = np.argsort(keys)
+    sk = keys[order]
+
+    def find(i, k, j, l):
+        for a, b, c, d in ((i, k, j, l), (k, i, j, l), (i, k, l, j), (k, i, l, j),
+                           (j, l, i, k), (l, j, i, k), (j, l, k, i), (l, j, k, i)):
+            kk = key(np.array([a]), c, b, d)[0]  # is = e1a, js = e2a, ks = e1b, ls = e2b
+            t = np.searchsorted(sk, kk)
+            if t < sk.size and sk[t] == kk:
+                return order[t]
+        raise KeyError((i, k, j, l))
+    return find
+
+
+def test_gauss_eri_shells_on_110_ao_molecule(engine):
+    """The full Schwarz run on the 110-AO molecule (shell kernels for its
+    s, p and d shell pairs): the reference's values of 48 stratified quartets,
+    screened quartets exactly zero (src/two_e_slater.F90:268-272), diagonal
+    quartets equal to precond, and agreement with the component kernels."""
+    bas = configs.random_molecule()
+    engine.set_basis(bas)
+    eri, pre, nscr = engine.gauss_eri()
+    st = engine.eri_stats()
+    assert any(k[0] == "shell" for k in st)
+    n = bas.n
+    is_, js, ks, ls = B.quartet_mapping(n)
+    g = golden("rand12_ng20_eri.npz")
+    find = _kcp_lookup(is_, js, ks, ls, n)
+    idx = np.array([find(*map(int, qq)) for qq in g["quartets"]])
+    np.testing.assert_allclose(eri[idx], g["eri"], rtol=1e-10, atol=1e-14)
+    scr = np.abs(pre[ks - 1, is_ - 1] * pre[ls - 1, js - 1]) < 1e-30
+    diag = (is_ == js) & (ks == ls)
+    assert nscr == int(scr.sum())
+    assert np.all(eri[scr & ~diag] == 0.0)
+    np.testing.assert_array_equal(eri[diag], pre[ks[diag] - 1, is_[diag] - 1])
+    rng = np.random.default_rng(11)
+    live = np.flatnonzero(~scr)
+    pick = rng.choice(live, 3000, replace=False)
+    a = engine.gauss_ijkl(is_[pick], ks[pick], js[pick], ls[pick])
+    scale = np.abs(eri).max()
+    np.testing.assert_allclose(eri[pick], a, rtol=1e-11, atol=1e-15 * scale)
+
+
 def test_errors_are_reported(engine):
     from slater_stoch_amd.zvmc import ZVError
     bas = configs.water_minimal(ng0=4)
