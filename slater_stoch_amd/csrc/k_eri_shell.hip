@@ -24,8 +24,10 @@
 // applies the Schwarz test per component quartet and writes triangle order.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "eri_device.h"
@@ -41,7 +43,8 @@ struct SPD {
     const int64_t *puoff; // AoS primitive data offset (doubles) per shell pair
     const int64_t *euoff; // AoS E offset (doubles) per shell pair
     const double *pu;     // [a][8]: p, 1/p, Px, Py, Pz, c, 0, 0
-    const double *eu;     // [a][component][h]
+    const double *eu;     // [a][component][h]  (scalar reads, direct mode)
+    const double *eu2;    // [a][h][component]  (lane-per-component reads, reduction mode)
 };
 
 struct ShellItem {
@@ -58,7 +61,8 @@ struct ShellGroup {
 };
 
 // AoS copies of a shell pair's primitive data and E (one block per shell pair)
-__global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, double *eu)
+__global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, double *eu,
+                                                    double *eu2)
 {
     const int s = blockIdx.x;
     const int c0 = sd.coff[s], nc = sd.coff[s + 1] - c0;
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
         }
         dp[t] = v;
     }
-    double *de = eu + sd.euoff[s];
+    double *de = eu + sd.euoff[s], *de2 = eu2 + sd.euoff[s];
     const int64_t tot = (int64_t)np * nc * H;
     for (int64_t t = threadIdx.x; t < tot; t += blockDim.x) {
         const int h = (int)(t % H);
@@ -89,16 +93,44 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
         const int cu = (int)(r % nc);
         const int a = (int)(r / nc);
         const int P = sd.comp[c0 + cu];
-        de[t] = pp.E[pp.eoff[P] + (int64_t)h * pp.cap[P] + a];
+        const double v = pp.E[pp.eoff[P] + (int64_t)h * pp.cap[P] + a];
+        de[t] = v;
+        de2[((int64_t)a * H + h) * nc + cu] = v;
     }
 }
 
-template <int LU, int LL>
-__global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd, const ShellItem *items,
-                                                  double *part)
+constexpr int pow2ceil(int x)
+{
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// The uniform-side arrays and the output are separate restrict arguments so
+// the compiler may read the former through the scalar cache (no store of
+// the kernel can reach them).  CLB lane-side components are processed per
+// pass: they share the Boys values and the R table of a primitive quartet.
+//
+// Two ways to finish a primitive quartet's contribution:
+//  * direct (RED = false): every lane contracts its X with the E of every
+//    uniform component (scalar E operands, nu * HU FMAs per lane);
+//  * reduction (RED = true): since the uniform E does not depend on the
+//    lane primitive pair, X is first summed over the 64 lanes through LDS,
+//    G_hu = sum_c X_hu(c), and lane u then contracts G with the E of its own
+//    uniform component u (HU FMAs per lane).  Used for nu >= ZV_RED_MIN.
+template <int LU, int LL, int CLB, bool RED>
+__global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
+                                                  const ShellItem *__restrict__ items,
+                                                  const double *__restrict__ pu_all,
+                                                  const double *__restrict__ eu_all,
+                                                  const double *__restrict__ eu2_all,
+                                                  double *__restrict__ part)
 {
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
-    __shared__ double accs[ZV_MAXU * 64];
+    constexpr int NR = CLB * HU;      // rows summed over the wave (reduction mode)
+    constexpr int S = NR | 1;         // odd lane stride of the transpose buffer
+    constexpr int LDSD = RED ? 64 * S + NR : ZV_MAXU * CLB * 64;
+    __shared__ double lds[LDSD];
     const int lane = threadIdx.x;
     const ShellItem it = items[blockIdx.x];
     const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
@@ -106,15 +138,28 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd, const ShellItem
     const int repl = sd.comp[lb];
     const int nl = pp.cnt[repl];
     const int64_t ol = pp.off[repl];
-    const double *pu = sd.pu + sd.puoff[it.su];
-    const double *eu = sd.eu + sd.euoff[it.su] + (int64_t)it.cu0 * HU;
+    const double *pu = pu_all + sd.puoff[it.su];
+    const double *eu = eu_all + sd.euoff[it.su] + (int64_t)it.cu0 * HU;
+    const double *eu2 = eu2_all + sd.euoff[it.su] + it.cu0 + lane;
     const int64_t erow = (int64_t)ncu * HU;
     const int nlc = it.cl1 - it.cl0;
-    for (int cl = it.cl0; cl < it.cl1; cl++) {
-        const int Pl = sd.comp[lb + cl];
-        const int64_t el = pp.eoff[Pl];
-        const int capl = pp.cap[Pl];
-        for (int u = 0; u < it.nu; u++) accs[u * 64 + lane] = 0.0;
+    const bool own = lane < it.nu; // reduction mode: lane owns uniform component cu0 + lane
+    for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
+        int64_t el[CLB];
+        int capl[CLB];
+        bool has[CLB];
+#pragma unroll
+        for (int j = 0; j < CLB; j++) {
+            has[j] = cl + j < it.cl1;
+            const int Pl = sd.comp[lb + (has[j] ? cl + j : cl)];
+            el[j] = pp.eoff[Pl];
+            capl[j] = pp.cap[Pl];
+        }
+        double acc[CLB];
+#pragma unroll
+        for (int j = 0; j < CLB; j++) acc[j] = 0.0;
+        if (!RED)
+            for (int u = 0; u < it.nu * CLB; u++) lds[u * 64 + lane] = 0.0;
         for (int c0 = 0; c0 < nl; c0 += 64) {
             const int c = c0 + lane;
             const bool live = c < nl;
@@ -122,12 +167,14 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd, const ShellItem
             const double q = pp.p[ol + cc], iq = pp.ip[ol + cc];
             const double Qx = pp.Px[ol + cc], Qy = pp.Py[ol + cc], Qz = pp.Pz[ol + cc];
             const double ck = live ? pp.c[ol + cc] : 0.0;
-            double El[HL];
+            double El[CLB][HL];
 #pragma unroll
-            for (int h = 0; h < HL; h++) {
-                const double e = pp.E[el + (int64_t)h * capl + cc];
-                El[h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -e : e;
-            }
+            for (int j = 0; j < CLB; j++)
+#pragma unroll
+                for (int h = 0; h < HL; h++) {
+                    const double e = has[j] ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                    El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -e : e;
+                }
             for (int a = it.a0; a < it.a1; a++) {
                 const double *pa = pu + (int64_t)a * 8;
                 const double p = pa[0], ip = pa[1];
@@ -141,30 +188,110 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd, const ShellItem
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
                 const double f = (pa[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
-                double Xv[HU];
+                // hl outer, hu inner: CLB * HU independent FMA chains
+                double Xv[CLB][HU];
 #pragma unroll
-                for (int hu = 0; hu < HU; hu++) {
-                    double w = 0.0;
+                for (int j = 0; j < CLB; j++)
 #pragma unroll
-                    for (int hl = 0; hl < HL; hl++)
-                        w = fma(El[hl],
-                                R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
-                                       kH.v[hu] + kH.v[hl])],
-                                w);
-                    Xv[hu] = w * f;
-                }
-                const double *ea = eu + (int64_t)a * erow;
-                for (int u = 0; u < it.nu; u++) {
-                    double y = 0.0;
+                    for (int hu = 0; hu < HU; hu++)
+                        Xv[j][hu] = El[j][0] * R[hu];
 #pragma unroll
-                    for (int hu = 0; hu < HU; hu++) y = fma(ea[u * HU + hu], Xv[hu], y);
-                    accs[u * 64 + lane] += y;
+                for (int hl = 1; hl < HL; hl++)
+#pragma unroll
+                    for (int hu = 0; hu < HU; hu++)
+#pragma unroll
+                        for (int j = 0; j < CLB; j++)
+                            Xv[j][hu] = fma(El[j][hl],
+                                            R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
+                                                   kH.v[hu] + kH.v[hl])],
+                                            Xv[j][hu]);
+#pragma unroll
+                for (int j = 0; j < CLB; j++)
+#pragma unroll
+                    for (int hu = 0; hu < HU; hu++) Xv[j][hu] *= f;
+                if constexpr (RED) {
+                    // the E of the owned uniform component, loaded while the
+                    // wave sums X through LDS
+                    double ev[HU];
+                    const double *ea = eu2 + (int64_t)a * HU * ncu;
+#pragma unroll
+                    for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
+#pragma unroll
+                    for (int j = 0; j < CLB; j++)
+#pragma unroll
+                        for (int hu = 0; hu < HU; hu++) lds[lane * S + j * HU + hu] = Xv[j][hu];
+                    __syncthreads();
+                    if constexpr (NR <= 64) {
+                        // P lanes per segment, 64 / P segments of each row
+                        constexpr int P = pow2ceil(NR), G = 64 / P, SEG = 64 / G;
+                        const int r = lane & (P - 1), k = lane / P;
+                        double t4[4] = {0.0, 0.0, 0.0, 0.0};
+                        if (r < NR) {
+#pragma unroll
+                            for (int i = 0; i < SEG; i++)
+                                t4[i & 3] += lds[(k * SEG + i) * S + r];
+                        }
+                        double v = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+#pragma unroll
+                        for (int o = P; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+                        if (k == 0 && r < NR) lds[64 * S + r] = v;
+                    } else {
+                        for (int r = lane; r < NR; r += 64) {
+                            double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                            for (int i = 0; i < 64; i++) t4[i & 3] += lds[i * S + r];
+                            lds[64 * S + r] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < CLB; j++) {
+                        double y[2] = {0.0, 0.0};
+#pragma unroll
+                        for (int hu = 0; hu < HU; hu++)
+                            y[hu & 1] = fma(ev[hu], lds[64 * S + j * HU + hu], y[hu & 1]);
+                        acc[j] += y[0] + y[1];
+                    }
+                } else {
+                    const double *ea = eu + (int64_t)a * erow;
+                    for (int u = 0; u < it.nu; u++) {
+                        // independent partial sums keep the FMA chains short
+                        constexpr int NP = CLB == 1 ? 8 : 4;
+                        double y[CLB][NP];
+#pragma unroll
+                        for (int j = 0; j < CLB; j++)
+#pragma unroll
+                            for (int t = 0; t < NP; t++) y[j][t] = 0.0;
+#pragma unroll
+                        for (int hu = 0; hu < HU; hu++) {
+                            const double e = ea[u * HU + hu];
+#pragma unroll
+                            for (int j = 0; j < CLB; j++)
+                                y[j][hu % NP] = fma(e, Xv[j][hu], y[j][hu % NP]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < CLB; j++) {
+                            double t = y[j][0];
+#pragma unroll
+                            for (int r = 1; r < NP; r++) t += y[j][r];
+                            lds[(u * CLB + j) * 64 + lane] += t;
+                        }
+                    }
                 }
             }
         }
-        for (int u = 0; u < it.nu; u++) {
-            const double v = wave_sum(accs[u * 64 + lane]);
-            if (lane == 0) part[it.part + (int64_t)u * nlc + (cl - it.cl0)] = v;
+        if constexpr (RED) {
+#pragma unroll
+            for (int j = 0; j < CLB; j++)
+                if (own && has[j]) part[it.part + (int64_t)lane * nlc + (cl + j - it.cl0)] = acc[j];
+        } else {
+            for (int u = 0; u < it.nu; u++)
+#pragma unroll
+                for (int j = 0; j < CLB; j++) {
+                    const double v = wave_sum(lds[(u * CLB + j) * 64 + lane]);
+                    if (lane == 0 && has[j])
+                        part[it.part + (int64_t)u * nlc + (cl + j - it.cl0)] = v;
+                }
         }
     }
 }
@@ -197,36 +324,55 @@ __global__ void k_zero(double *x, int64_t n)
     if (i < n) x[i] = 0.0;
 }
 
+// Largest lane-component batch of each (LU, LL) class: two wherever the
+// shared R table and the doubled E / X registers fit one wave per SIMD
+// without spilling.  Items with a single lane component use batch 1.
+constexpr int kShellClb[5][5] = {{2, 2, 2, 2, 2},
+                                 {2, 2, 2, 2, 2},
+                                 {2, 2, 2, 2, 1},
+                                 {2, 2, 2, 2, 1},
+                                 {2, 2, 2, 1, 1}};
+
 template <int LU, int LL>
-void launch_shell_t(unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
-                    const ShellItem *items, double *part)
+void launch_shell_t(int clb, bool red, unsigned grid, hipStream_t st, const PP &pp,
+                    const SPD &sd, const ShellItem *items, double *part)
 {
-    hipLaunchKernelGGL((k_eri_shell<LU, LL>), dim3(grid), dim3(64), 0, st, pp, sd, items, part);
+#define ZV_SHELL_LAUNCH(C, R)                                                                     \
+    hipLaunchKernelGGL((k_eri_shell<LU, LL, C, R>), dim3(grid), dim3(64), 0, st, pp, sd, items, \
+                       sd.pu, sd.eu, sd.eu2, part)
+    if (clb == 2) {
+        if (red) ZV_SHELL_LAUNCH(2, true);
+        else ZV_SHELL_LAUNCH(2, false);
+    } else {
+        if (red) ZV_SHELL_LAUNCH(1, true);
+        else ZV_SHELL_LAUNCH(1, false);
+    }
+#undef ZV_SHELL_LAUNCH
 }
 
 template <int LU>
-bool launch_shell_u(int LL, unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
-                    const ShellItem *items, double *part)
+bool launch_shell_u(int LL, int clb, bool red, unsigned grid, hipStream_t st, const PP &pp,
+                    const SPD &sd, const ShellItem *items, double *part)
 {
     switch (LL) {
-    case 0: launch_shell_t<LU, 0>(grid, st, pp, sd, items, part); return true;
-    case 1: launch_shell_t<LU, 1>(grid, st, pp, sd, items, part); return true;
-    case 2: launch_shell_t<LU, 2>(grid, st, pp, sd, items, part); return true;
-    case 3: launch_shell_t<LU, 3>(grid, st, pp, sd, items, part); return true;
-    case 4: launch_shell_t<LU, 4>(grid, st, pp, sd, items, part); return true;
+    case 0: launch_shell_t<LU, 0>(clb, red, grid, st, pp, sd, items, part); return true;
+    case 1: launch_shell_t<LU, 1>(clb, red, grid, st, pp, sd, items, part); return true;
+    case 2: launch_shell_t<LU, 2>(clb, red, grid, st, pp, sd, items, part); return true;
+    case 3: launch_shell_t<LU, 3>(clb, red, grid, st, pp, sd, items, part); return true;
+    case 4: launch_shell_t<LU, 4>(clb, red, grid, st, pp, sd, items, part); return true;
     }
     return false;
 }
 
-bool launch_shell(int LU, int LL, unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
-                  const ShellItem *items, double *part)
+bool launch_shell(int LU, int LL, int clb, bool red, unsigned grid, hipStream_t st,
+                  const PP &pp, const SPD &sd, const ShellItem *items, double *part)
 {
     switch (LU) {
-    case 0: return launch_shell_u<0>(LL, grid, st, pp, sd, items, part);
-    case 1: return launch_shell_u<1>(LL, grid, st, pp, sd, items, part);
-    case 2: return launch_shell_u<2>(LL, grid, st, pp, sd, items, part);
-    case 3: return launch_shell_u<3>(LL, grid, st, pp, sd, items, part);
-    case 4: return launch_shell_u<4>(LL, grid, st, pp, sd, items, part);
+    case 0: return launch_shell_u<0>(LL, clb, red, grid, st, pp, sd, items, part);
+    case 1: return launch_shell_u<1>(LL, clb, red, grid, st, pp, sd, items, part);
+    case 2: return launch_shell_u<2>(LL, clb, red, grid, st, pp, sd, items, part);
+    case 3: return launch_shell_u<3>(LL, clb, red, grid, st, pp, sd, items, part);
+    case 4: return launch_shell_u<4>(LL, clb, red, grid, st, pp, sd, items, part);
     }
     return false;
 }
@@ -241,39 +387,74 @@ double boys_r_flops(int L)
     return 25.0 + 3.0 * (L + 1) + 4.0 * (HR4 - 1.0) + 12.0;
 }
 
-// Contraction flops per primitive quartet, lane component and uniform range of nu
-double contract_flops(int LU, int LL, int nu)
+// X contraction and scaling per primitive quartet and lane component
+double x_flops(int LU, int LL) { return 2.0 * hc(LU) * hc(LL) + hc(LU); }
+
+// uniform components per item and the smallest count that uses the
+// reduction mode
+constexpr int kMaxUDirect = ZV_MAXU, kMaxURed = 64, kRedMin = 4;
+
+// Estimated kernel cost, in flop-equivalents per primitive quartet and pass
+// of clb lane components, of finishing with nu uniform components: the
+// direct mode's scalar-operand FMAs run about three times slower than
+// register-only work; the reduction mode costs an LDS transpose and sum.
+double finish_cost(int LU, int clb, int nu, bool red)
 {
-    double HU = hc(LU), HL = hc(LL);
-    return 2.0 * HU * HL + HU + 2.0 * nu * HU;
+    const double HU = hc(LU), NR = clb * HU;
+    return red ? 4.0 * NR + 128.0 * std::ceil(NR / 64.0) : 3.0 * 2.0 * nu * NR;
 }
 
 struct Plan {
-    std::vector<ShellItem> items; // class k = 5 LU + LL at [lo[k], lo[k + 1])
-    int lo[26] = {};
+    // bucket b = 25 (2 (clb - 1) + red) + 5 LU + LL at [lo[b], lo[b + 1])
+    std::vector<ShellItem> items;
+    int lo[101] = {};
     std::vector<ShellGroup> groups;
     int64_t npart = 0;
     double units[25] = {}, flops[25] = {};
 };
 
+// lane components run in batches of two (even part) and one (odd remainder)
+inline int paired(int LU, int LL, int ncl) { return kShellClb[LU][LL] == 2 ? ncl & ~1 : 0; }
+
+// uniform-component ranges of one orientation: (cu0, nu, red)
+void uniform_ranges(int ncu, std::vector<std::tuple<int, int, bool>> &out)
+{
+    out.clear();
+    const bool red = ncu >= kRedMin;
+    const int step = red ? kMaxURed : kMaxUDirect;
+    for (int cu0 = 0; cu0 < ncu; cu0 += step) {
+        const int nu = std::min(step, ncu - cu0);
+        out.emplace_back(cu0, nu, red && nu >= kRedMin);
+    }
+}
+
 // Kernel-time cost model of (U|L): lanes padded to 64, Boys and R redone per
-// lane component and uniform range.
+// pass of lane components and per uniform range.
 double item_cost(const zv_context *c, int su, int sl)
 {
     const int LU = c->h_sp_L[su], LL = c->h_sp_L[sl];
     const int ncu = c->h_sp_coff[su + 1] - c->h_sp_coff[su];
     const int ncl = c->h_sp_coff[sl + 1] - c->h_sp_coff[sl];
     const double npu = c->h_sp_np[su], lanes = std::ceil(c->h_sp_np[sl] / 64.0) * 64.0;
+    const int n2 = paired(LU, LL, ncl);
+    std::vector<std::tuple<int, int, bool>> ur;
+    uniform_ranges(ncu, ur);
     double cost = 0.0;
-    for (int cu0 = 0; cu0 < ncu; cu0 += ZV_MAXU) {
-        const int nu = std::min(ZV_MAXU, ncu - cu0);
-        cost += lanes * npu * ncl * (boys_r_flops(LU + LL) + contract_flops(LU, LL, nu));
+    for (const auto &r : ur) {
+        const int nu = std::get<1>(r);
+        const bool red = std::get<2>(r);
+        const double br = boys_r_flops(LU + LL);
+        cost += lanes * npu *
+                (n2 / 2 * (br + 2 * x_flops(LU, LL) + finish_cost(LU, 2, nu, red)) +
+                 (ncl - n2) * (br + x_flops(LU, LL) + finish_cost(LU, 1, nu, red)));
     }
     return cost;
 }
 
 // Orientation, splitting and ordering of a list of shell quartets (x, y),
-// both shell pairs of order <= ZV_LMAX_T with kept primitive pairs.
+// both shell pairs of order <= ZV_LMAX_T with kept primitive pairs.  Items
+// are cut along lane-component passes first (no extra work), then along
+// uniform primitive-pair slices (partial sums added by the scatter).
 void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq, Plan &pl)
 {
     struct Or {
@@ -291,52 +472,85 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         total += o.cost;
     }
     const double budget = std::max(total / 40000.0, 2.0e6);
-    std::vector<std::pair<double, ShellItem>> tmp[25];
+    std::vector<std::pair<double, ShellItem>> tmp[100];
+    std::vector<std::tuple<int, int, bool>> ur;
     for (const Or &o : ors) {
         const int LU = c->h_sp_L[o.su], LL = c->h_sp_L[o.sl], L = LU + LL;
         const int ncu = c->h_sp_coff[o.su + 1] - c->h_sp_coff[o.su];
         const int ncl = c->h_sp_coff[o.sl + 1] - c->h_sp_coff[o.sl];
         const int npu = c->h_sp_np[o.su], npl = c->h_sp_np[o.sl];
+        const int n2 = paired(LU, LL, ncl);
         const double lanes = std::ceil(npl / 64.0) * 64.0;
+        const double chunks = lanes / 64.0;
         const int k = 5 * LU + LL;
-        for (int cu0 = 0; cu0 < ncu; cu0 += ZV_MAXU) {
-            const int nu = std::min(ZV_MAXU, ncu - cu0);
-            const double w = lanes * npu * (boys_r_flops(L) + contract_flops(LU, LL, nu));
-            const int k1 = (int)std::min<double>(ncl, std::max(1.0, std::ceil(w * ncl / budget)));
-            for (int r = 0; r < k1; r++) {
-                const int l0 = (int)((int64_t)ncl * r / k1), l1 = (int)((int64_t)ncl * (r + 1) / k1);
-                const int nlr = l1 - l0;
-                const double cr = w * nlr;
-                const int ns = (int)std::min<double>(npu, std::max(1.0, std::ceil(cr / budget)));
-                ShellGroup g{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pl.npart};
-                pl.groups.push_back(g);
-                for (int s = 0; s < ns; s++) {
-                    const int a0 = (int)((int64_t)npu * s / ns), a1 = (int)((int64_t)npu * (s + 1) / ns);
-                    ShellItem it{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
-                                 pl.npart + (int64_t)s * nu * nlr};
-                    tmp[k].push_back({cr * (a1 - a0) / npu, it});
-                    const double pq = (double)(a1 - a0) * npl;
-                    pl.units[k] += pq * nlr;
-                    pl.flops[k] += pq * (boys_r_flops(L) + nlr * contract_flops(LU, LL, nu));
+        const double HU = hc(LU);
+        uniform_ranges(ncu, ur);
+        for (const auto &ru : ur) {
+            const int cu0 = std::get<0>(ru), nu = std::get<1>(ru);
+            const bool red = std::get<2>(ru);
+            // segment [0, n2) in passes of two components, [n2, ncl) in passes of one
+            for (int seg = 0; seg < 2; seg++) {
+                const int clb = seg == 0 ? 2 : 1;
+                const int s0 = seg == 0 ? 0 : n2, s1 = seg == 0 ? n2 : ncl;
+                const int npass = (s1 - s0) / clb;
+                if (npass <= 0) continue;
+                const double wp = lanes * npu *
+                                  (boys_r_flops(L) + clb * x_flops(LU, LL) +
+                                   finish_cost(LU, clb, nu, red)); // per pass
+                const int k1 = (int)std::min<double>(
+                    npass, std::max(1.0, std::ceil(npass * wp / budget)));
+                const int b = 25 * (2 * (clb - 1) + (red ? 1 : 0)) + k;
+                for (int r = 0; r < k1; r++) {
+                    const int p0 = (int)((int64_t)npass * r / k1);
+                    const int p1 = (int)((int64_t)npass * (r + 1) / k1);
+                    const int l0 = s0 + p0 * clb, l1 = s0 + p1 * clb;
+                    const int nlr = l1 - l0;
+                    const double cr = (p1 - p0) * wp;
+                    const int ns =
+                        (int)std::min<double>(npu, std::max(1.0, std::ceil(cr / budget)));
+                    ShellGroup g{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pl.npart};
+                    pl.groups.push_back(g);
+                    for (int sl = 0; sl < ns; sl++) {
+                        const int a0 = (int)((int64_t)npu * sl / ns);
+                        const int a1 = (int)((int64_t)npu * (sl + 1) / ns);
+                        ShellItem it{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
+                                     pl.npart + (int64_t)sl * nu * nlr};
+                        tmp[b].push_back({cr * (a1 - a0) / npu, it});
+                        // algorithmic flops: Boys and R once per primitive
+                        // quartet; per lane component the X contraction, then
+                        // either nu direct dot products per primitive quartet
+                        // or the wave sum per primitive quartet plus nu dot
+                        // products per uniform primitive and lane chunk
+                        const double pq = (double)(a1 - a0) * npl;
+                        pl.units[k] += pq * nlr;
+                        double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
+                        if (red)
+                            fl += nlr * (pq * HU + (a1 - a0) * chunks * 2.0 * nu * HU);
+                        else
+                            fl += nlr * pq * 2.0 * nu * HU;
+                        pl.flops[k] += fl;
+                    }
+                    pl.npart += (int64_t)ns * nu * nlr;
                 }
-                pl.npart += (int64_t)ns * nu * nlr;
             }
         }
     }
     pl.items.clear();
-    for (int k = 0; k < 25; k++) {
+    for (int k = 0; k < 100; k++) {
         pl.lo[k] = (int)pl.items.size();
         std::stable_sort(tmp[k].begin(), tmp[k].end(),
                          [](const std::pair<double, ShellItem> &a,
                             const std::pair<double, ShellItem> &b) { return a.first > b.first; });
         for (const auto &e : tmp[k]) pl.items.push_back(e.second);
     }
-    pl.lo[25] = (int)pl.items.size();
+    pl.lo[100] = (int)pl.items.size();
 }
 
 } // namespace
 
 // ====================================================================== host
+static SPD spd_of(zv_context *c);
+
 // Shell-pair tables on the device and the AoS copies of the uniform side.
 int zv_shell_ready(zv_context *c, const PP &pp)
 {
@@ -357,7 +571,9 @@ int zv_shell_ready(zv_context *c, const PP &pp)
         eu_tot += (int64_t)np * nc * hc(c->h_sp_L[s]);
     }
     ZV_CHECK(zv_ensure(c, c->b_spu, sizeof(double) * std::max<int64_t>(pu_tot, 1)));
-    ZV_CHECK(zv_ensure(c, c->b_spe, sizeof(double) * std::max<int64_t>(eu_tot, 1)));
+    // two layouts of the uniform-side E: [a][comp][h] then [a][h][comp]
+    ZV_CHECK(zv_ensure(c, c->b_spe, sizeof(double) * std::max<int64_t>(2 * eu_tot, 1)));
+    c->sp_eu_total = eu_tot;
     int64_t *d_puoff = (int64_t *)c->b_spmeta.p;
     int64_t *d_euoff = d_puoff + nsp;
     int *d_comp = (int *)(d_euoff + nsp);
@@ -370,12 +586,12 @@ int zv_shell_ready(zv_context *c, const PP &pp)
                                   hipMemcpyHostToDevice, c->stream));
     ZV_HIPCHECK(c, hipMemcpyAsync(d_coff, c->h_sp_coff.data(), sizeof(int) * (nsp + 1),
                                   hipMemcpyHostToDevice, c->stream));
-    SPD sd{d_comp, d_coff, d_puoff, d_euoff, (const double *)c->b_spu.p,
-           (const double *)c->b_spe.p};
+    SPD sd = spd_of(c);
     {
         PhaseTimer tm(c, PH_PRIM);
         hipLaunchKernelGGL(k_shellpairs, dim3(nsp), dim3(256), 0, c->stream, pp, sd,
-                           (double *)c->b_spu.p, (double *)c->b_spe.p);
+                           (double *)c->b_spu.p, (double *)c->b_spe.p,
+                           (double *)c->b_spe.p + eu_tot);
         ZV_HIPCHECK(c, hipGetLastError());
     }
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -389,8 +605,9 @@ static SPD spd_of(zv_context *c)
     int64_t *d_euoff = d_puoff + c->nsp;
     int *d_comp = (int *)(d_euoff + c->nsp);
     int *d_coff = d_comp + c->npair;
-    return SPD{d_comp, d_coff, d_puoff, d_euoff, (const double *)c->b_spu.p,
-               (const double *)c->b_spe.p};
+    const double *eu = (const double *)c->b_spe.p;
+    return SPD{d_comp, d_coff, d_puoff, d_euoff, (const double *)c->b_spu.p, eu,
+               eu + c->sp_eu_total};
 }
 
 // Plan and launch the shell kernels of one Schwarz phase over the shell
@@ -417,17 +634,23 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
     const SPD sd = spd_of(c);
     const ShellItem *di = (const ShellItem *)bitem.p;
     for (int k = 0; k < 25; k++) {
-        const int n = pl.lo[k + 1] - pl.lo[k];
-        if (!n) continue;
         const int LU = k / 5, LL = k % 5;
         const int cls = 36 + 6 * LU + LL;
+        int nk = 0;
+        for (int v = 0; v < 4; v++) nk += pl.lo[25 * v + k + 1] - pl.lo[25 * v + k];
+        if (!nk) continue;
         c->cls_pq[cls] += pl.units[k];
         c->cls_flops[cls] += pl.flops[k];
         PhaseTimer tm(c, PH_ERI, cls);
-        if (!launch_shell(LU, LL, (unsigned)n, c->stream, pp, sd, di + pl.lo[k],
-                          (double *)bpart.p))
-            return zv_fail(c, ZV_ERR_STATE, "no shell kernel for (%d,%d)", LU, LL);
-        ZV_HIPCHECK(c, hipGetLastError());
+        for (int v = 0; v < 4; v++) {
+            const int b = 25 * v + k;
+            const int n = pl.lo[b + 1] - pl.lo[b];
+            if (!n) continue;
+            if (!launch_shell(LU, LL, v / 2 + 1, (v & 1) != 0, (unsigned)n, c->stream, pp, sd,
+                              di + pl.lo[b], (double *)bpart.p))
+                return zv_fail(c, ZV_ERR_STATE, "no shell kernel for (%d,%d)", LU, LL);
+            ZV_HIPCHECK(c, hipGetLastError());
+        }
     }
     return ZV_OK;
 }

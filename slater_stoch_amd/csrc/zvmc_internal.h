@@ -80,6 +80,7 @@ struct zv_context {
     DevBuf b_spmeta;                   // comp[npair], coff[nsp+1], puoff[nsp], euoff[nsp]
     DevBuf b_spu, b_spe;               // per primitive pair: p,1/p,P,c; E [a][comp][h]
     std::vector<int> h_sp_np;          // kept primitive pairs per shell pair
+    int64_t sp_eu_total = 0;           // doubles of one uniform-side E layout
     bool sp_ready = false;
     // planned shell work of the two Schwarz phases (host copies kept alive for the copies)
     DevBuf b_sitem1, b_sgroup1, b_part1, b_sitem2, b_sgroup2, b_part2, b_plist;
