@@ -99,38 +99,23 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
     }
 }
 
-constexpr int pow2ceil(int x)
-{
-    int p = 1;
-    while (p < x) p <<= 1;
-    return p;
-}
-
 // The uniform-side arrays and the output are separate restrict arguments so
 // the compiler may read the former through the scalar cache (no store of
 // the kernel can reach them).  CLB lane-side components are processed per
 // pass: they share the Boys values and the R table of a primitive quartet.
-//
-// Two ways to finish a primitive quartet's contribution:
-//  * direct (RED = false): every lane contracts its X with the E of every
-//    uniform component (scalar E operands, nu * HU FMAs per lane);
-//  * reduction (RED = true): since the uniform E does not depend on the
-//    lane primitive pair, X is first summed over the 64 lanes through LDS,
-//    G_hu = sum_c X_hu(c), and lane u then contracts G with the E of its own
-//    uniform component u (HU FMAs per lane).  Used for nu >= ZV_RED_MIN.
-template <int LU, int LL, int CLB, bool RED>
+
+// Direct mode (few uniform components): every lane contracts its X with the
+// E of every uniform component, scalar E operands, nu * HU FMAs per lane and
+// primitive quartet; per-lane sums over the uniform components live in LDS.
+template <int LU, int LL, int CLB>
 __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                                                   const ShellItem *__restrict__ items,
                                                   const double *__restrict__ pu_all,
                                                   const double *__restrict__ eu_all,
-                                                  const double *__restrict__ eu2_all,
                                                   double *__restrict__ part)
 {
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
-    constexpr int NR = CLB * HU;      // rows summed over the wave (reduction mode)
-    constexpr int S = NR | 1;         // odd lane stride of the transpose buffer
-    constexpr int LDSD = RED ? 64 * S + NR : ZV_MAXU * CLB * 64;
-    __shared__ double lds[LDSD];
+    __shared__ double accs[ZV_MAXU * CLB * 64];
     const int lane = threadIdx.x;
     const ShellItem it = items[blockIdx.x];
     const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
@@ -140,10 +125,8 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
     const int64_t ol = pp.off[repl];
     const double *pu = pu_all + sd.puoff[it.su];
     const double *eu = eu_all + sd.euoff[it.su] + (int64_t)it.cu0 * HU;
-    const double *eu2 = eu2_all + sd.euoff[it.su] + it.cu0 + lane;
     const int64_t erow = (int64_t)ncu * HU;
     const int nlc = it.cl1 - it.cl0;
-    const bool own = lane < it.nu; // reduction mode: lane owns uniform component cu0 + lane
     for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
         int64_t el[CLB];
         int capl[CLB];
@@ -155,11 +138,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
             el[j] = pp.eoff[Pl];
             capl[j] = pp.cap[Pl];
         }
-        double acc[CLB];
-#pragma unroll
-        for (int j = 0; j < CLB; j++) acc[j] = 0.0;
-        if (!RED)
-            for (int u = 0; u < it.nu * CLB; u++) lds[u * 64 + lane] = 0.0;
+        for (int u = 0; u < it.nu * CLB; u++) accs[u * 64 + lane] = 0.0;
         for (int c0 = 0; c0 < nl; c0 += 64) {
             const int c = c0 + lane;
             const bool live = c < nl;
@@ -209,90 +188,187 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 for (int j = 0; j < CLB; j++)
 #pragma unroll
                     for (int hu = 0; hu < HU; hu++) Xv[j][hu] *= f;
-                if constexpr (RED) {
-                    // the E of the owned uniform component, loaded while the
-                    // wave sums X through LDS
-                    double ev[HU];
-                    const double *ea = eu2 + (int64_t)a * HU * ncu;
-#pragma unroll
-                    for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
+                const double *ea = eu + (int64_t)a * erow;
+                for (int u = 0; u < it.nu; u++) {
+                    // independent partial sums keep the FMA chains short
+                    constexpr int NP = CLB == 1 ? 8 : 4;
+                    double y[CLB][NP];
 #pragma unroll
                     for (int j = 0; j < CLB; j++)
 #pragma unroll
-                        for (int hu = 0; hu < HU; hu++) lds[lane * S + j * HU + hu] = Xv[j][hu];
-                    __syncthreads();
-                    if constexpr (NR <= 64) {
-                        // P lanes per segment, 64 / P segments of each row
-                        constexpr int P = pow2ceil(NR), G = 64 / P, SEG = 64 / G;
-                        const int r = lane & (P - 1), k = lane / P;
-                        double t4[4] = {0.0, 0.0, 0.0, 0.0};
-                        if (r < NR) {
+                        for (int t = 0; t < NP; t++) y[j][t] = 0.0;
 #pragma unroll
-                            for (int i = 0; i < SEG; i++)
-                                t4[i & 3] += lds[(k * SEG + i) * S + r];
-                        }
-                        double v = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-#pragma unroll
-                        for (int o = P; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
-                        if (k == 0 && r < NR) lds[64 * S + r] = v;
-                    } else {
-                        for (int r = lane; r < NR; r += 64) {
-                            double t4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                            for (int i = 0; i < 64; i++) t4[i & 3] += lds[i * S + r];
-                            lds[64 * S + r] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-                        }
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int j = 0; j < CLB; j++) {
-                        double y[2] = {0.0, 0.0};
-#pragma unroll
-                        for (int hu = 0; hu < HU; hu++)
-                            y[hu & 1] = fma(ev[hu], lds[64 * S + j * HU + hu], y[hu & 1]);
-                        acc[j] += y[0] + y[1];
-                    }
-                } else {
-                    const double *ea = eu + (int64_t)a * erow;
-                    for (int u = 0; u < it.nu; u++) {
-                        // independent partial sums keep the FMA chains short
-                        constexpr int NP = CLB == 1 ? 8 : 4;
-                        double y[CLB][NP];
+                    for (int hu = 0; hu < HU; hu++) {
+                        const double e = ea[u * HU + hu];
 #pragma unroll
                         for (int j = 0; j < CLB; j++)
+                            y[j][hu % NP] = fma(e, Xv[j][hu], y[j][hu % NP]);
+                    }
 #pragma unroll
-                            for (int t = 0; t < NP; t++) y[j][t] = 0.0;
+                    for (int j = 0; j < CLB; j++) {
+                        double t = y[j][0];
 #pragma unroll
-                        for (int hu = 0; hu < HU; hu++) {
-                            const double e = ea[u * HU + hu];
-#pragma unroll
-                            for (int j = 0; j < CLB; j++)
-                                y[j][hu % NP] = fma(e, Xv[j][hu], y[j][hu % NP]);
-                        }
-#pragma unroll
-                        for (int j = 0; j < CLB; j++) {
-                            double t = y[j][0];
-#pragma unroll
-                            for (int r = 1; r < NP; r++) t += y[j][r];
-                            lds[(u * CLB + j) * 64 + lane] += t;
-                        }
+                        for (int r = 1; r < NP; r++) t += y[j][r];
+                        accs[(u * CLB + j) * 64 + lane] += t;
                     }
                 }
             }
         }
-        if constexpr (RED) {
+        for (int u = 0; u < it.nu; u++)
+#pragma unroll
+            for (int j = 0; j < CLB; j++) {
+                const double v = wave_sum(accs[(u * CLB + j) * 64 + lane]);
+                if (lane == 0 && has[j])
+                    part[it.part + (int64_t)u * nlc + (cl + j - it.cl0)] = v;
+            }
+    }
+}
+
+constexpr int pow2ceil(int x)
+{
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Reduction mode (nu >= 4): the uniform E does not depend on the lane
+// primitive pair, so for each uniform primitive pair a the wave first forms
+// X_hu = sum over all lane primitive pairs (every lane chunk, the prefactor
+// folded into the lane E), sums it over the 64 lanes through LDS, and lane u
+// then contracts the sum with the E of its own uniform component u.  One LDS
+// reduction per (a, pass); the uniform E of lane u is loaded before the
+// chunk loop when the registers allow, so its latency hides behind it.
+template <int LU, int LL, int CLB>
+__global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
+                                                      const ShellItem *__restrict__ items,
+                                                      const double *__restrict__ pu_all,
+                                                      const double *__restrict__ eu2_all,
+                                                      double *__restrict__ part)
+{
+    constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
+    constexpr int NR = CLB * HU; // rows summed over the wave
+    constexpr int S = NR | 1;    // odd lane stride of the transpose buffer
+    constexpr bool PREF = HR + CLB * (HL + HU) + HU <= 200;
+    __shared__ double lds[64 * S + NR];
+    const int lane = threadIdx.x;
+    const ShellItem it = items[blockIdx.x];
+    const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
+    const int ncu = sd.coff[it.su + 1] - ub;
+    const int repl = sd.comp[lb];
+    const int nl = pp.cnt[repl];
+    const int64_t ol = pp.off[repl];
+    const double *pu = pu_all + sd.puoff[it.su];
+    const double *eu2 = eu2_all + sd.euoff[it.su] + it.cu0 + lane;
+    const int nlc = it.cl1 - it.cl0;
+    const bool own = lane < it.nu; // lane owns uniform component cu0 + lane
+    for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
+        int64_t el[CLB];
+        int capl[CLB];
+        bool has[CLB];
+#pragma unroll
+        for (int j = 0; j < CLB; j++) {
+            has[j] = cl + j < it.cl1;
+            const int Pl = sd.comp[lb + (has[j] ? cl + j : cl)];
+            el[j] = pp.eoff[Pl];
+            capl[j] = pp.cap[Pl];
+        }
+        double acc[CLB];
+#pragma unroll
+        for (int j = 0; j < CLB; j++) acc[j] = 0.0;
+        for (int a = it.a0; a < it.a1; a++) {
+            const double *pa = pu + (int64_t)a * 8;
+            const double p = pa[0], ip = pa[1], Px = pa[2], Py = pa[3], Pz = pa[4], cb = pa[5];
+            const double *ea = eu2 + (int64_t)a * HU * ncu;
+            double ev[HU];
+            if constexpr (PREF) {
+#pragma unroll
+                for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
+            }
+            double Xa[CLB][HU];
 #pragma unroll
             for (int j = 0; j < CLB; j++)
-                if (own && has[j]) part[it.part + (int64_t)lane * nlc + (cl + j - it.cl0)] = acc[j];
-        } else {
-            for (int u = 0; u < it.nu; u++)
 #pragma unroll
-                for (int j = 0; j < CLB; j++) {
-                    const double v = wave_sum(lds[(u * CLB + j) * 64 + lane]);
-                    if (lane == 0 && has[j])
-                        part[it.part + (int64_t)u * nlc + (cl + j - it.cl0)] = v;
+                for (int hu = 0; hu < HU; hu++) Xa[j][hu] = 0.0;
+            for (int c0 = 0; c0 < nl; c0 += 64) {
+                const int c = c0 + lane;
+                const bool live = c < nl;
+                const int cc = live ? c : c0; // in-range index for idle lanes
+                const double q = pp.p[ol + cc], iq = pp.ip[ol + cc];
+                const double X = Px - pp.Px[ol + cc], Y = Py - pp.Py[ol + cc],
+                             Z = Pz - pp.Pz[ol + cc];
+                const double ck = live ? pp.c[ol + cc] : 0.0;
+                const double s = p + q;
+                const double is = 1.0 / s;
+                const double alpha = (p * q) * is;
+                const double T = alpha * ((X * X + Y * Y) + Z * Z);
+                double F[L + 1];
+                boys<L>(T, F);
+                double R[HR];
+                hermite_R<L>(alpha, X, Y, Z, F, R);
+                const double f = (cb * ck) * (((kPi52 * ip) * iq) * sqrt(is));
+                double El[CLB][HL];
+#pragma unroll
+                for (int j = 0; j < CLB; j++)
+#pragma unroll
+                    for (int h = 0; h < HL; h++) {
+                        const double e =
+                            has[j] ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                        El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -(e * f) : e * f;
+                    }
+#pragma unroll
+                for (int hl = 0; hl < HL; hl++)
+#pragma unroll
+                    for (int hu = 0; hu < HU; hu++)
+#pragma unroll
+                        for (int j = 0; j < CLB; j++)
+                            Xa[j][hu] = fma(El[j][hl],
+                                            R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
+                                                   kH.v[hu] + kH.v[hl])],
+                                            Xa[j][hu]);
+            }
+            if constexpr (!PREF) {
+#pragma unroll
+                for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < CLB; j++)
+#pragma unroll
+                for (int hu = 0; hu < HU; hu++) lds[lane * S + j * HU + hu] = Xa[j][hu];
+            __syncthreads();
+            if constexpr (NR <= 64) {
+                // P lanes per segment, 64 / P segments of each row
+                constexpr int P = pow2ceil(NR), G = 64 / P, SEG = 64 / G;
+                const int r = lane & (P - 1), k = lane / P;
+                double t4[4] = {0.0, 0.0, 0.0, 0.0};
+                if (r < NR) {
+#pragma unroll
+                    for (int i = 0; i < SEG; i++) t4[i & 3] += lds[(k * SEG + i) * S + r];
                 }
+                double v = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+#pragma unroll
+                for (int o = P; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+                if (k == 0 && r < NR) lds[64 * S + r] = v;
+            } else {
+                for (int r = lane; r < NR; r += 64) {
+                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int i = 0; i < 64; i++) t4[i & 3] += lds[i * S + r];
+                    lds[64 * S + r] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < CLB; j++) {
+                double y[2] = {0.0, 0.0};
+#pragma unroll
+                for (int hu = 0; hu < HU; hu++)
+                    y[hu & 1] = fma(ev[hu], lds[64 * S + j * HU + hu], y[hu & 1]);
+                acc[j] += y[0] + y[1];
+            }
         }
+#pragma unroll
+        for (int j = 0; j < CLB; j++)
+            if (own && has[j]) part[it.part + (int64_t)lane * nlc + (cl + j - it.cl0)] = acc[j];
     }
 }
 
@@ -324,30 +400,40 @@ __global__ void k_zero(double *x, int64_t n)
     if (i < n) x[i] = 0.0;
 }
 
-// Largest lane-component batch of each (LU, LL) class: two wherever the
-// shared R table and the doubled E / X registers fit one wave per SIMD
-// without spilling.  Items with a single lane component use batch 1.
+// Largest lane-component batch of each (LU, LL) class, direct and reduction
+// mode: two wherever the shared R table and the doubled lane registers fit
+// one wave per SIMD without spilling.  Items with a single lane component
+// use batch 1.
 constexpr int kShellClb[5][5] = {{2, 2, 2, 2, 2},
                                  {2, 2, 2, 2, 2},
                                  {2, 2, 2, 2, 1},
                                  {2, 2, 2, 2, 1},
                                  {2, 2, 2, 1, 1}};
+constexpr int kShellClbRed[5][5] = {{2, 2, 2, 2, 2},
+                                    {2, 2, 2, 2, 1},
+                                    {2, 2, 2, 2, 1},
+                                    {2, 2, 2, 1, 1},
+                                    {2, 2, 1, 1, 1}};
 
 template <int LU, int LL>
 void launch_shell_t(int clb, bool red, unsigned grid, hipStream_t st, const PP &pp,
                     const SPD &sd, const ShellItem *items, double *part)
 {
-#define ZV_SHELL_LAUNCH(C, R)                                                                     \
-    hipLaunchKernelGGL((k_eri_shell<LU, LL, C, R>), dim3(grid), dim3(64), 0, st, pp, sd, items, \
-                       sd.pu, sd.eu, sd.eu2, part)
-    if (clb == 2) {
-        if (red) ZV_SHELL_LAUNCH(2, true);
-        else ZV_SHELL_LAUNCH(2, false);
+    if (red) {
+        if (clb == 2)
+            hipLaunchKernelGGL((k_eri_shell_red<LU, LL, 2>), dim3(grid), dim3(64), 0, st, pp, sd,
+                               items, sd.pu, sd.eu2, part);
+        else
+            hipLaunchKernelGGL((k_eri_shell_red<LU, LL, 1>), dim3(grid), dim3(64), 0, st, pp, sd,
+                               items, sd.pu, sd.eu2, part);
     } else {
-        if (red) ZV_SHELL_LAUNCH(1, true);
-        else ZV_SHELL_LAUNCH(1, false);
+        if (clb == 2)
+            hipLaunchKernelGGL((k_eri_shell<LU, LL, 2>), dim3(grid), dim3(64), 0, st, pp, sd,
+                               items, sd.pu, sd.eu, part);
+        else
+            hipLaunchKernelGGL((k_eri_shell<LU, LL, 1>), dim3(grid), dim3(64), 0, st, pp, sd,
+                               items, sd.pu, sd.eu, part);
     }
-#undef ZV_SHELL_LAUNCH
 }
 
 template <int LU>
@@ -394,14 +480,16 @@ double x_flops(int LU, int LL) { return 2.0 * hc(LU) * hc(LL) + hc(LU); }
 // reduction mode
 constexpr int kMaxUDirect = ZV_MAXU, kMaxURed = 64, kRedMin = 4;
 
-// Estimated kernel cost, in flop-equivalents per primitive quartet and pass
-// of clb lane components, of finishing with nu uniform components: the
-// direct mode's scalar-operand FMAs run about three times slower than
-// register-only work; the reduction mode costs an LDS transpose and sum.
-double finish_cost(int LU, int clb, int nu, bool red)
+// Estimated kernel cost of finishing, in lane flop-equivalents: the direct
+// mode per primitive quartet and pass (scalar-operand FMAs run about three
+// times slower than register-only work); the reduction mode per uniform
+// primitive pair and pass (LDS transpose, row sums, two barriers and the
+// load of the uniform E, mostly latency at one wave per SIMD).
+double finish_direct(int LU, int clb, int nu) { return 3.0 * 2.0 * nu * clb * hc(LU); }
+double finish_red(int LU, int clb)
 {
-    const double HU = hc(LU), NR = clb * HU;
-    return red ? 4.0 * NR + 128.0 * std::ceil(NR / 64.0) : 3.0 * 2.0 * nu * NR;
+    const double NR = clb * hc(LU);
+    return 64.0 * (4.0 * NR + 128.0 * std::ceil(NR / 64.0)) + 40000.0;
 }
 
 struct Plan {
@@ -414,7 +502,10 @@ struct Plan {
 };
 
 // lane components run in batches of two (even part) and one (odd remainder)
-inline int paired(int LU, int LL, int ncl) { return kShellClb[LU][LL] == 2 ? ncl & ~1 : 0; }
+inline int paired(int LU, int LL, int ncl, bool red)
+{
+    return (red ? kShellClbRed[LU][LL] : kShellClb[LU][LL]) == 2 ? ncl & ~1 : 0;
+}
 
 // uniform-component ranges of one orientation: (cu0, nu, red)
 void uniform_ranges(int ncu, std::vector<std::tuple<int, int, bool>> &out)
@@ -430,23 +521,27 @@ void uniform_ranges(int ncu, std::vector<std::tuple<int, int, bool>> &out)
 
 // Kernel-time cost model of (U|L): lanes padded to 64, Boys and R redone per
 // pass of lane components and per uniform range.
+double pass_cost(int LU, int LL, int clb, int nu, bool red, double npu, double lanes)
+{
+    const double work = lanes * npu * (boys_r_flops(LU + LL) + clb * x_flops(LU, LL));
+    return work + (red ? npu * finish_red(LU, clb) : lanes * npu * finish_direct(LU, clb, nu));
+}
+
 double item_cost(const zv_context *c, int su, int sl)
 {
     const int LU = c->h_sp_L[su], LL = c->h_sp_L[sl];
     const int ncu = c->h_sp_coff[su + 1] - c->h_sp_coff[su];
     const int ncl = c->h_sp_coff[sl + 1] - c->h_sp_coff[sl];
     const double npu = c->h_sp_np[su], lanes = std::ceil(c->h_sp_np[sl] / 64.0) * 64.0;
-    const int n2 = paired(LU, LL, ncl);
     std::vector<std::tuple<int, int, bool>> ur;
     uniform_ranges(ncu, ur);
     double cost = 0.0;
     for (const auto &r : ur) {
         const int nu = std::get<1>(r);
         const bool red = std::get<2>(r);
-        const double br = boys_r_flops(LU + LL);
-        cost += lanes * npu *
-                (n2 / 2 * (br + 2 * x_flops(LU, LL) + finish_cost(LU, 2, nu, red)) +
-                 (ncl - n2) * (br + x_flops(LU, LL) + finish_cost(LU, 1, nu, red)));
+        const int n2 = paired(LU, LL, ncl, red);
+        cost += n2 / 2 * pass_cost(LU, LL, 2, nu, red, npu, lanes) +
+                (ncl - n2) * pass_cost(LU, LL, 1, nu, red, npu, lanes);
     }
     return cost;
 }
@@ -479,24 +574,21 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         const int ncu = c->h_sp_coff[o.su + 1] - c->h_sp_coff[o.su];
         const int ncl = c->h_sp_coff[o.sl + 1] - c->h_sp_coff[o.sl];
         const int npu = c->h_sp_np[o.su], npl = c->h_sp_np[o.sl];
-        const int n2 = paired(LU, LL, ncl);
         const double lanes = std::ceil(npl / 64.0) * 64.0;
-        const double chunks = lanes / 64.0;
         const int k = 5 * LU + LL;
         const double HU = hc(LU);
         uniform_ranges(ncu, ur);
         for (const auto &ru : ur) {
             const int cu0 = std::get<0>(ru), nu = std::get<1>(ru);
             const bool red = std::get<2>(ru);
+            const int n2 = paired(LU, LL, ncl, red);
             // segment [0, n2) in passes of two components, [n2, ncl) in passes of one
             for (int seg = 0; seg < 2; seg++) {
                 const int clb = seg == 0 ? 2 : 1;
                 const int s0 = seg == 0 ? 0 : n2, s1 = seg == 0 ? n2 : ncl;
                 const int npass = (s1 - s0) / clb;
                 if (npass <= 0) continue;
-                const double wp = lanes * npu *
-                                  (boys_r_flops(L) + clb * x_flops(LU, LL) +
-                                   finish_cost(LU, clb, nu, red)); // per pass
+                const double wp = pass_cost(LU, LL, clb, nu, red, npu, lanes); // per pass
                 const int k1 = (int)std::min<double>(
                     npass, std::max(1.0, std::ceil(npass * wp / budget)));
                 const int b = 25 * (2 * (clb - 1) + (red ? 1 : 0)) + k;
@@ -519,13 +611,13 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
                         // algorithmic flops: Boys and R once per primitive
                         // quartet; per lane component the X contraction, then
                         // either nu direct dot products per primitive quartet
-                        // or the wave sum per primitive quartet plus nu dot
-                        // products per uniform primitive and lane chunk
+                        // or the sum over lane primitives per primitive
+                        // quartet plus nu dot products per uniform primitive
                         const double pq = (double)(a1 - a0) * npl;
                         pl.units[k] += pq * nlr;
                         double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
                         if (red)
-                            fl += nlr * (pq * HU + (a1 - a0) * chunks * 2.0 * nu * HU);
+                            fl += nlr * (pq * HU + (a1 - a0) * 2.0 * nu * HU);
                         else
                             fl += nlr * pq * 2.0 * nu * HU;
                         pl.flops[k] += fl;
