@@ -35,6 +35,7 @@ TARGET = 1e-6
 NBL = 10
 PILOT = 4000
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 
 
 def job(eng, bas, prec, seed):
@@ -63,36 +64,65 @@ def job(eng, bas, prec, seed):
 
 
 def kernel_name(key):
-    kind, a, b = key
-    return f"k_eri_shell<{a}, {b}>" if kind == "shell" else (
-        f"k_eri_class<{a}, {b}>" if a < 5 else "k_eri_generic")
+    if key[0] == "shell":
+        _, lu, ll, clb, red = key
+        return f"k_eri_shell{'_red' if red else ''}<{lu}, {ll}, {clb}>"
+    _, a, b = key
+    return f"k_eri_class<{a}, {b}>" if a < 5 else "k_eri_generic"
 
 
-def roofline_of(res):
-    """The heaviest ERI kernel class of the last step: algorithmic FP64 flops
+def _traffic(name):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/kernel_traffic.json)."""
+    tf = ROOT / "profiles" / "kernel_traffic.json"
+    if not tf.exists():
+        return None
+    try:
+        return json.loads(tf.read_text()).get(name)
+    except Exception:
+        return None
+
+
+def roofline_eri(res):
+    """The heaviest ERI kernel of the last step: algorithmic FP64 flops
     (Boys values and R table once per primitive quartet, Hermite contractions
     per component, as counted by the library) over its measured kernel time
     (HIP events on the library's stream)."""
     st = res["eri_stats"]
     key, d = max(st.items(), key=lambda kv: kv[1]["ms"])
     name = kernel_name(key)
-    sec = d["ms"] / 1e3
-    achieved = d["flops"] / sec / 1e12
-    traffic = None
-    tf = ROOT / "profiles" / "eri_traffic.json"
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get(name)
-        except Exception:
-            traffic = None
+    achieved = d["flops"] / (d["ms"] / 1e3) / 1e12
     total_flops = sum(v["flops"] for v in st.values())
     total_ms = sum(v["ms"] for v in st.values())
     return {"bound": "fp64-valu", "kernel": name, "achieved": achieved,
             "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-            "traffic": traffic, "launches": d["launches"],
-            "avg_launch_ms": d["ms"] / max(1, d["launches"]),
+            "traffic": _traffic(name), "launches": d["launches"],
+            "total_ms": d["ms"], "avg_launch_ms": d["ms"] / max(1, d["launches"]),
             "flops_per_launch": d["flops"] / max(1, d["launches"]),
-            "all_eri_classes_tflops": total_flops / (total_ms / 1e3) / 1e12}
+            "all_eri_kernels_tflops": total_flops / (total_ms / 1e3) / 1e12}
+
+
+def roofline_estimator(res, nint, prec):
+    """The tiled Monte Carlo estimator: 13 algorithmic flops per
+    quartet-sample (3 differences, r^2, the two density products, the
+    accumulation) plus one reciprocal square root, over the estimator
+    kernel time measured with HIP events.  One launch per block."""
+    ms, n = res["mc_timings"]["estimator"]
+    qs = float(nint) * NBL * res["production_npts"]
+    achieved = 13.0 * qs / (ms / 1e3) / 1e12
+    peak = PEAK_FP64 if prec != "fp32" else PEAK_FP32
+    name = "k_est_tile<double>" if prec != "fp32" else "k_est_tile<float>"
+    return {"bound": "fp64-valu" if prec != "fp32" else "fp32-valu", "kernel": name,
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": _traffic(name), "launches": n, "total_ms": ms,
+            "avg_launch_ms": ms / max(1, n), "flops_per_launch": 13.0 * qs / max(1, n),
+            "quartet_samples_per_s": qs / (ms / 1e3)}
+
+
+def roofline_of(res, nint):
+    """The dominant kernel of the step (largest total kernel time)."""
+    eri, est = roofline_eri(res), roofline_estimator(res, nint, "fp64")
+    return (est, eri) if est["total_ms"] >= eri["total_ms"] else (eri, est)
 
 
 def cpu_baseline(bas, samples, nscr, budget_s=20.0):
@@ -185,7 +215,7 @@ def main():
         print(f"[bench] fp32 step: {res32['total']:.2f} s (MC {res32['mc']:.2f} s)",
               file=sys.stderr, flush=True)
     last = res64[-1]
-    roof = roofline_of(last)
+    roof, roof_other = roofline_of(last, bas.nint)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(bas, last["samples"], last["n_screened"])
@@ -220,6 +250,7 @@ def main():
                                      if est_ms > 0 else None),
         "mfma_fraction": 0.0,
         "roofline": roof,
+        "roofline_other_phase": roof_other,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -125,11 +125,12 @@ int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
  * n[k] = number of launches of that phase.  Arrays of 8. */
 int zv_last_timings(const zv_context *ctx, double *ms, int64_t *launches);
 
-/* Per ERI kernel class of the last zv_gauss_eri / zv_gauss_ijkl_list call,
- * arrays of 72.  Entries 0..35: component kernels, index 6*LB + LK (LB >= LK
- * the Hermite orders of the two AO pairs, 5 = the runtime-order kernel);
- * entries 36..71: shell kernels, index 36 + 6*LU + LL (orders of the
- * uniform-side and lane-side shell pairs).  ms: kernel milliseconds;
+/* Per ERI kernel of the last zv_gauss_eri / zv_gauss_ijkl_list call, arrays
+ * of 136.  Entries 0..35: component kernels, index 6*LB + LK (LB >= LK the
+ * Hermite orders of the two AO pairs, 5 = the runtime-order kernel);
+ * entries 36..135: shell kernels, index 36 + 25*v + 5*LU + LL (LU, LL the
+ * orders of the uniform-side and lane-side shell pairs; v = 2*(lane batch-1)
+ * + mode, mode 0 direct, 1 wave-sum reduction).  ms: kernel milliseconds;
  * launches; units: primitive-quartet evaluations (times the lane-side
  * components for shell kernels); flops: algorithmic FP64 flops (Boys values
  * and R table once per primitive quartet, contractions per component).

@@ -498,7 +498,7 @@ struct Plan {
     int lo[101] = {};
     std::vector<ShellGroup> groups;
     int64_t npart = 0;
-    double units[25] = {}, flops[25] = {};
+    double units[100] = {}, flops[100] = {}; // per bucket
 };
 
 // lane components run in batches of two (even part) and one (odd remainder)
@@ -614,13 +614,13 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
                         // or the sum over lane primitives per primitive
                         // quartet plus nu dot products per uniform primitive
                         const double pq = (double)(a1 - a0) * npl;
-                        pl.units[k] += pq * nlr;
+                        pl.units[b] += pq * nlr;
                         double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
                         if (red)
                             fl += nlr * (pq * HU + (a1 - a0) * 2.0 * nu * HU);
                         else
                             fl += nlr * pq * 2.0 * nu * HU;
-                        pl.flops[k] += fl;
+                        pl.flops[b] += fl;
                     }
                     pl.npart += (int64_t)ns * nu * nlr;
                 }
@@ -725,24 +725,19 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
                                   c->stream));
     const SPD sd = spd_of(c);
     const ShellItem *di = (const ShellItem *)bitem.p;
-    for (int k = 0; k < 25; k++) {
-        const int LU = k / 5, LL = k % 5;
-        const int cls = 36 + 6 * LU + LL;
-        int nk = 0;
-        for (int v = 0; v < 4; v++) nk += pl.lo[25 * v + k + 1] - pl.lo[25 * v + k];
-        if (!nk) continue;
-        c->cls_pq[cls] += pl.units[k];
-        c->cls_flops[cls] += pl.flops[k];
+    // bucket b = 25 v + 5 LU + LL, v = 2 (clb - 1) + red: statistics slot 36 + b
+    for (int b = 0; b < 100; b++) {
+        const int n = pl.lo[b + 1] - pl.lo[b];
+        if (!n) continue;
+        const int v = b / 25, k = b % 25, LU = k / 5, LL = k % 5;
+        const int cls = 36 + b;
+        c->cls_pq[cls] += pl.units[b];
+        c->cls_flops[cls] += pl.flops[b];
         PhaseTimer tm(c, PH_ERI, cls);
-        for (int v = 0; v < 4; v++) {
-            const int b = 25 * v + k;
-            const int n = pl.lo[b + 1] - pl.lo[b];
-            if (!n) continue;
-            if (!launch_shell(LU, LL, v / 2 + 1, (v & 1) != 0, (unsigned)n, c->stream, pp, sd,
-                              di + pl.lo[b], (double *)bpart.p))
-                return zv_fail(c, ZV_ERR_STATE, "no shell kernel for (%d,%d)", LU, LL);
-            ZV_HIPCHECK(c, hipGetLastError());
-        }
+        if (!launch_shell(LU, LL, v / 2 + 1, (v & 1) != 0, (unsigned)n, c->stream, pp, sd,
+                          di + pl.lo[b], (double *)bpart.p))
+            return zv_fail(c, ZV_ERR_STATE, "no shell kernel for (%d,%d)", LU, LL);
+        ZV_HIPCHECK(c, hipGetLastError());
     }
     return ZV_OK;
 }

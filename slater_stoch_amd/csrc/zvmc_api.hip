@@ -75,7 +75,7 @@ void zv_reset_timings(zv_context *c)
         c->t_ms[p] = 0.0;
         c->t_n[p] = 0;
     }
-    for (int k = 0; k < 72; k++) {
+    for (int k = 0; k < ZV_NSTAT; k++) {
         c->cls_ms[k] = 0.0;
         c->cls_n[k] = 0;
         c->cls_pq[k] = 0.0;
@@ -96,7 +96,7 @@ int zv_collect_timings(zv_context *c)
             c->t_ms[c->pend_phase[t]] += ms;
             c->t_n[c->pend_phase[t]] += 1;
             int k = c->pend_cls[t];
-            if (k >= 0 && k < 72) {
+            if (k >= 0 && k < ZV_NSTAT) {
                 c->cls_ms[k] += ms;
                 c->cls_n[k] += 1;
             }
@@ -239,7 +239,7 @@ extern "C" int zv_last_eri_stats(const zv_context *c, double *ms, int64_t *launc
                                  double *units, double *flops)
 {
     if (!c) return ZV_ERR_ARG;
-    for (int k = 0; k < 72; k++) {
+    for (int k = 0; k < ZV_NSTAT; k++) {
         if (ms) ms[k] = c->cls_ms[k];
         if (launches) launches[k] = c->cls_n[k];
         if (units) units[k] = c->cls_pq[k];

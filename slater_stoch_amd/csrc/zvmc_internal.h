@@ -19,6 +19,7 @@
 #define ZV_LPAIR_MAX 8 // largest pair angular momentum supported at all
 #define ZV_SHMAX 15    // largest shell (Cartesian components sharing one Gaussian fit)
 #define ZV_MAXU 36     // components of the uniform side handled by one shell work item
+#define ZV_NSTAT 136   // ERI statistics slots (component and shell kernel variants)
 
 struct DevBuf {
     void *p = nullptr;
@@ -95,11 +96,12 @@ struct zv_context {
     double t_ms[PH_N] = {};
     int64_t t_n[PH_N] = {};
     std::vector<int> pend_cls;        // ERI class of each pending timer (-1: none)
-    // ERI statistics: [0, 36) component kernels 6 LB + LK, [36, 72) shell kernels
-    double cls_ms[72] = {};
-    int64_t cls_n[72] = {};
-    double cls_pq[72] = {};           // primitive-quartet evaluations per class
-    double cls_flops[72] = {};        // algorithmic FP64 flops per class
+    // ERI statistics: [0, 36) component kernels 6 LB + LK; [36, 136) shell
+    // kernels 36 + 25 v + 5 LU + LL, v = 2 (lane batch - 1) + reduction mode
+    double cls_ms[ZV_NSTAT] = {};
+    int64_t cls_n[ZV_NSTAT] = {};
+    double cls_pq[ZV_NSTAT] = {};     // primitive-quartet evaluations per class
+    double cls_flops[ZV_NSTAT] = {};  // algorithmic FP64 flops per class
     std::vector<int> h_cnt;           // kept primitive pairs per AO pair (host copy)
     void *blas = nullptr;             // rocBLAS handle (cleanup GEMMs)
 };

@@ -202,19 +202,24 @@ class Engine:
         return e, int(nrem[0])
 
     def eri_stats(self):
-        """Per kernel class of the last gauss_eri / gauss_ijkl: {(kind, LB, LK):
-        dict(ms, launches, units, flops)} with kind "component" (orders of the
-        two AO pairs) or "shell" (orders of the uniform and lane shell pairs)."""
-        ms, un, fl = np.zeros(72), np.zeros(72), np.zeros(72)
-        n = np.zeros(72, np.int64)
+        """Per ERI kernel of the last gauss_eri / gauss_ijkl: {key: dict(ms,
+        launches, units, flops)} with key ("component", LB, LK) or ("shell",
+        LU, LL, lane_batch, reduction)."""
+        n_stat = 136
+        ms, un, fl = np.zeros(n_stat), np.zeros(n_stat), np.zeros(n_stat)
+        n = np.zeros(n_stat, np.int64)
         self._check(self.lib.zv_last_eri_stats(self.h, _p(ms), _p(n), _p(un), _p(fl)))
         out = {}
-        for k in range(72):
-            if n[k] or un[k]:
-                kind = "shell" if k >= 36 else "component"
-                j = k - 36 if k >= 36 else k
-                out[(kind, j // 6, j % 6)] = dict(ms=float(ms[k]), launches=int(n[k]),
-                                                  units=float(un[k]), flops=float(fl[k]))
+        for k in range(n_stat):
+            if not (n[k] or un[k]):
+                continue
+            if k < 36:
+                key = ("component", k // 6, k % 6)
+            else:
+                v, c = divmod(k - 36, 25)
+                key = ("shell", c // 5, c % 5, v // 2 + 1, bool(v & 1))
+            out[key] = dict(ms=float(ms[k]), launches=int(n[k]), units=float(un[k]),
+                            flops=float(fl[k]))
         return out
 
     def timings(self):
