@@ -46,6 +46,7 @@ def job(eng, bas, prec, seed):
     eri_stats = eng.eri_stats()
     an_timings = eng.timings()
     mean, err = eng.mc_blocks(NBL, PILOT, prec, seed=seed, stream=0)
+    pilot = (mean.copy(), err.copy())
     samples = NBL * PILOT
     emax = float(err.max())
     npts = PILOT
@@ -60,7 +61,39 @@ def job(eng, bas, prec, seed):
     t2 = time.perf_counter()
     return dict(total=t2 - t0, analytic=t1 - t0, mc=t2 - t1, max_stderr=emax, samples=samples,
                 production_npts=npts, n_screened=nscr, eri_stats=eri_stats,
-                an_timings=an_timings, mc_timings=mc_timings, eri=eri, mean=mean)
+                an_timings=an_timings, mc_timings=mc_timings, eri=eri, mean=mean, pilot=pilot)
+
+
+def precision_report(bas, m64, e64, m32):
+    """FP32-estimator error against the FP64 estimator on the same samples:
+    per-quartet mean differences, overall and binned by the number of
+    distinct nuclei of the quartet (the classes of src/compare_nuclei.F90)."""
+    from slater_stoch_amd import basis as B
+    q = B.quartet_mapping(bas.n)
+    cls = B.nuclei_class(bas, *q)
+    d = np.abs(m32 - m64)
+    out = {"quartets": int(d.size), "max_abs": float(d.max()),
+           "rms": float(np.sqrt(np.mean(d * d))),
+           "max_abs_over_fp64_stderr": float(np.max(d / np.maximum(e64, 1e-300))),
+           "by_distinct_nuclei": {}}
+    for k in range(1, 5):
+        sel = cls == k
+        if sel.any():
+            dk = d[sel]
+            out["by_distinct_nuclei"][str(k)] = {"quartets": int(sel.sum()),
+                                                 "max_abs": float(dk.max()),
+                                                 "rms": float(np.sqrt(np.mean(dk * dk)))}
+    return out
+
+
+def water_precision(eng):
+    """The same report on the 7-AO water molecule (10 x 4000 samples)."""
+    from slater_stoch_amd import configs
+    w = configs.water_minimal()
+    eng.set_basis(w)
+    m64, e64 = eng.mc_blocks(NBL, PILOT, "fp64", seed=3, stream=0)
+    m32, _ = eng.mc_blocks(NBL, PILOT, "fp32", seed=3, stream=0)
+    return precision_report(w, m64, e64, m32)
 
 
 def kernel_name(key):
@@ -216,6 +249,15 @@ def main():
               file=sys.stderr, flush=True)
     last = res64[-1]
     roof, roof_other = roofline_of(last, bas.nint)
+    precision = None
+    if res32 is not None:
+        # step 1 (seed 1) and the FP32 job share the pilot samples
+        precision = {"c2": precision_report(bas, res64[0]["pilot"][0], res64[0]["pilot"][1],
+                                            res32["pilot"][0]),
+                     "c1_water": water_precision(eng)}
+        print(f"[bench] fp32 vs fp64 estimator on the same samples: max |diff| "
+              f"{precision['c2']['max_abs']:.2e} Ha (110 AOs), "
+              f"{precision['c1_water']['max_abs']:.2e} Ha (water)", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(bas, last["samples"], last["n_screened"])
@@ -246,6 +288,7 @@ def main():
         "mc_s": float(np.mean([r["mc"] for r in res64])),
         "mixed_precision_s": None if res32 is None else res32["total"],
         "mixed_precision_max_stderr": None if res32 is None else res32["max_stderr"],
+        "mixed_precision_error": precision,
         "estimator_qsamples_per_s": (bas.nint * NBL * last["production_npts"] / (est_ms / 1e3)
                                      if est_ms > 0 else None),
         "mfma_fraction": 0.0,
