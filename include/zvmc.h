@@ -119,6 +119,14 @@ int zv_svd_clean(zv_context *ctx, int rank, int q, uint64_t seed, double *eri,
  * same matrix, src/davidson_clean.F90:30-120 (dense branch). */
 int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
 
+/* Progress lines of the long-running calls, written to stdout: level 1
+ * prints what the reference prints unconditionally in svd_clean ("Vectors
+ * a -- b / N" per randomized pass, src/svd_clean.F90:153-154, and the
+ * largest/smallest singular value of the pass, :285); level 0 (the default)
+ * prints nothing.  No counterpart in the reference, whose prints cannot be
+ * switched off. */
+int zv_set_verbose(zv_context *ctx, int level);
+
 /* Milliseconds spent in the last call's kernels, by phase:
  * t[0] sampler, t[1] densities, t[2] estimator, t[3] block statistics,
  * t[4] primitive pairs, t[5] analytic ERIs, t[6] copies, t[7] cleanup;

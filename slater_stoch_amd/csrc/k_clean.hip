@@ -261,6 +261,11 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
     for (uint32_t pass = 0; npass < N; pass++) {
         if (pass >= max_pass)
             return zv_fail(c, ZV_ERR_STATE, "svd_clean: no convergence after %u passes", pass);
+        if (c->verbose) {
+            printf(" Vectors %lld -- %lld / %lld\n", (long long)npass + 1,
+                   (long long)std::min<int64_t>(npass + r, N), (long long)N);
+            fflush(stdout);
+        }
         hipLaunchKernelGGL(k_randn, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, (int64_t)nr,
                            seed, pass, P);
         hipLaunchKernelGGL(k_colnormalize, dim3(r), dim3(256), 0, c->stream, N, P);
@@ -293,6 +298,11 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
                                       c->stream));
         ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
         if (hinfo != 0) return zv_fail(c, ZV_ERR_HIP, "%d: SVD failed", (int)hinfo);
+        if (c->verbose) {
+            printf(" Largest/Smallest singular value : %g %g\n", (double)(float)hD[0],
+                   (double)(float)hD[r - 1]);
+            fflush(stdout);
+        }
         // src/svd_clean.F90:286-296
         npass += r;
         double dmax = 0.0;

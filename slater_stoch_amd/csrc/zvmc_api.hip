@@ -57,13 +57,14 @@ PhaseTimer::PhaseTimer(zv_context *c_, Phase p, int cls_) : c(c_), ph(p), cls(cl
     };
     a = take();
     b = take();
-    if (a) hipEventRecord(a, c->stream);
+    // timing is best effort: a failed record only drops this measurement
+    if (a && hipEventRecord(a, c->stream) != hipSuccess) a = nullptr;
 }
 
 PhaseTimer::~PhaseTimer()
 {
     if (!a || !b) return;
-    hipEventRecord(b, c->stream);
+    if (hipEventRecord(b, c->stream) != hipSuccess) return;
     c->pend_phase.push_back(ph);
     c->pend_cls.push_back(cls);
     c->pend_ev.emplace_back(a, b);
@@ -186,8 +187,9 @@ static void free_basis(zv_context *c)
     void *ptrs[] = {c->d_center, c->d_zeta, c->d_c1, c->d_c2, c->d_g, c->d_npower, c->d_nsto,
                     c->d_itype, c->d_ngauss, c->d_pair_i, c->d_pair_k, c->d_pair_f,
                     c->d_pair_jf, c->d_pair_G, c->d_kbase};
+    // teardown is best effort: a failing free has nothing left to report to
     for (void *p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     c->d_center = c->d_zeta = c->d_c1 = c->d_c2 = c->d_g = nullptr;
     c->d_npower = c->d_nsto = c->d_itype = c->d_ngauss = nullptr;
     c->d_pair_i = c->d_pair_k = nullptr;
@@ -200,8 +202,8 @@ static void free_basis(zv_context *c)
 extern "C" int zv_destroy(zv_context *c)
 {
     if (!c) return ZV_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     free_basis(c);
     zv_blas_destroy(c);
     DevBuf *bufs[] = {&c->b_r1, &c->b_r2, &c->b_pi0, &c->b_pd, &c->b_accS, &c->b_accG,
@@ -210,9 +212,9 @@ extern "C" int zv_destroy(zv_context *c)
                       &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
                       &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist};
     for (DevBuf *b : bufs)
-        if (b->p) hipFree(b->p);
-    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    hipStreamDestroy(c->stream);
+        if (b->p) (void)hipFree(b->p);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return ZV_OK;
 }
@@ -222,6 +224,13 @@ extern "C" int zv_last_error(const zv_context *c, char *buf, int len)
     if (!buf || len <= 0) return ZV_ERR_ARG;
     const char *m = c ? c->err.c_str() : "no context";
     snprintf(buf, len, "%s", m);
+    return ZV_OK;
+}
+
+extern "C" int zv_set_verbose(zv_context *c, int level)
+{
+    if (!c) return ZV_ERR_ARG;
+    c->verbose = level;
     return ZV_OK;
 }
 

@@ -30,6 +30,7 @@ enum Phase { PH_SAMPLE = 0, PH_DENS, PH_EST, PH_STATS, PH_PRIM, PH_ERI, PH_COPY,
 
 struct zv_context {
     int device = 0;
+    int verbose = 0;
     hipStream_t stream = nullptr;
     std::string err;
 

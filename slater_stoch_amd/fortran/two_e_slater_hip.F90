@@ -161,6 +161,8 @@ program two_e_slater_hip
 
   ! ---------------------------------------------------------------- device
   call zv_check(ctx, zv_create(0_c_int, ctx), 'zv_create')
+  ! the reference prints the cleanup's progress lines unconditionally
+  call zv_check(ctx, zv_set_verbose(ctx, 1_c_int), 'zv_set_verbose')
   call zv_check(ctx, zv_set_basis(ctx, n, center, npower, n_sto, i_type, zeta, n_gauss, ldg, &
       c_gauss, g_gauss, a_zv), 'zv_set_basis')
   nint = zv_nint(n)

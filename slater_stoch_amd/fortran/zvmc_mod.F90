@@ -97,6 +97,11 @@ module zvmc
       real(c_double), intent(inout) :: eri(*)
       integer(c_int), intent(out) :: n_removed
     end function
+    integer(c_int) function zv_set_verbose(ctx, level) bind(C, name="zv_set_verbose")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+    end function
   end interface
 
 contains

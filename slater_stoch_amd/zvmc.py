@@ -42,6 +42,7 @@ SIGNATURES = {
     "zv_pair_samples": (I32, [VP, I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "zv_svd_clean": (I32, [VP, I32, I32, U64, VP, VP]),
     "zv_davidson_clean": (I32, [VP, VP, VP]),
+    "zv_set_verbose": (I32, [VP, I32]),
     "zv_last_timings": (I32, [VP, VP, VP]),
     "zv_last_eri_stats": (I32, [VP, VP, VP, VP, VP]),
 }
@@ -91,7 +92,7 @@ def build_mapping(n):
 class Engine:
     """One context on one gfx950 device (one HIP stream)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, verbose=0):
         self.lib = load_library()
         h = VP()
         rc = self.lib.zv_create(device, ctypes.byref(h))
@@ -99,6 +100,8 @@ class Engine:
             raise ZVError(rc, "no usable gfx950 device" if rc == ZV_ERR_NODEV else "zv_create")
         self.h = h
         self.basis = None
+        if verbose:
+            self._check(self.lib.zv_set_verbose(self.h, int(verbose)))
 
     def close(self):
         if getattr(self, "h", None):
