@@ -24,10 +24,8 @@
 // applies the Schwarz test per component quartet and writes triangle order.
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <numeric>
-#include <tuple>
 #include <vector>
 
 #include "eri_device.h"
@@ -507,17 +505,11 @@ inline int paired(int LU, int LL, int ncl, bool red)
     return (red ? kShellClbRed[LU][LL] : kShellClb[LU][LL]) == 2 ? ncl & ~1 : 0;
 }
 
-// uniform-component ranges of one orientation: (cu0, nu, red)
-void uniform_ranges(int ncu, std::vector<std::tuple<int, int, bool>> &out)
-{
-    out.clear();
-    const bool red = ncu >= kRedMin;
-    const int step = red ? kMaxURed : kMaxUDirect;
-    for (int cu0 = 0; cu0 < ncu; cu0 += step) {
-        const int nu = std::min(step, ncu - cu0);
-        out.emplace_back(cu0, nu, red && nu >= kRedMin);
-    }
-}
+// uniform-component ranges of one orientation: ranges of kMaxURed (reduction
+// mode, from kRedMin components on) or kMaxUDirect components; a short tail
+// of a reduction-mode pair below kRedMin runs in direct mode
+inline int uniform_step(int ncu) { return ncu >= kRedMin ? kMaxURed : kMaxUDirect; }
+inline bool uniform_red(int ncu, int nu) { return ncu >= kRedMin && nu >= kRedMin; }
 
 // Kernel-time cost model of (U|L): lanes padded to 64, Boys and R redone per
 // pass of lane components and per uniform range.
@@ -533,12 +525,11 @@ double item_cost(const zv_context *c, int su, int sl)
     const int ncu = c->h_sp_coff[su + 1] - c->h_sp_coff[su];
     const int ncl = c->h_sp_coff[sl + 1] - c->h_sp_coff[sl];
     const double npu = c->h_sp_np[su], lanes = std::ceil(c->h_sp_np[sl] / 64.0) * 64.0;
-    std::vector<std::tuple<int, int, bool>> ur;
-    uniform_ranges(ncu, ur);
+    const int step = uniform_step(ncu);
     double cost = 0.0;
-    for (const auto &r : ur) {
-        const int nu = std::get<1>(r);
-        const bool red = std::get<2>(r);
+    for (int cu0 = 0; cu0 < ncu; cu0 += step) {
+        const int nu = std::min(step, ncu - cu0);
+        const bool red = uniform_red(ncu, nu);
         const int n2 = paired(LU, LL, ncl, red);
         cost += n2 / 2 * pass_cost(LU, LL, 2, nu, red, npu, lanes) +
                 (ncl - n2) * pass_cost(LU, LL, 1, nu, red, npu, lanes);
@@ -566,9 +557,11 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         ors.push_back(o);
         total += o.cost;
     }
+    // items of at most 1/40000 of the total cost: larger quartets are split,
+    // so the sorted queue ends on short items
     const double budget = std::max(total / 40000.0, 2.0e6);
-    std::vector<std::pair<double, ShellItem>> tmp[100];
-    std::vector<std::tuple<int, int, bool>> ur;
+    std::vector<ShellItem> tmp[100];
+    std::vector<float> tcost[100];
     for (const Or &o : ors) {
         const int LU = c->h_sp_L[o.su], LL = c->h_sp_L[o.sl], L = LU + LL;
         const int ncu = c->h_sp_coff[o.su + 1] - c->h_sp_coff[o.su];
@@ -577,10 +570,10 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         const double lanes = std::ceil(npl / 64.0) * 64.0;
         const int k = 5 * LU + LL;
         const double HU = hc(LU);
-        uniform_ranges(ncu, ur);
-        for (const auto &ru : ur) {
-            const int cu0 = std::get<0>(ru), nu = std::get<1>(ru);
-            const bool red = std::get<2>(ru);
+        const int step = uniform_step(ncu);
+        for (int cu0 = 0; cu0 < ncu; cu0 += step) {
+            const int nu = std::min(step, ncu - cu0);
+            const bool red = uniform_red(ncu, nu);
             const int n2 = paired(LU, LL, ncl, red);
             // segment [0, n2) in passes of two components, [n2, ncl) in passes of one
             for (int seg = 0; seg < 2; seg++) {
@@ -602,40 +595,53 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
                         (int)std::min<double>(npu, std::max(1.0, std::ceil(cr / budget)));
                     ShellGroup g{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pl.npart};
                     pl.groups.push_back(g);
+                    // algorithmic flops: Boys and R once per primitive quartet;
+                    // per lane component the X contraction, then either nu
+                    // direct dot products per primitive quartet or the sum over
+                    // lane primitives per primitive quartet plus nu dot
+                    // products per uniform primitive
+                    const double pq = (double)npu * npl;
+                    pl.units[b] += pq * nlr;
+                    double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
+                    fl += red ? nlr * (pq * HU + (double)npu * 2.0 * nu * HU)
+                              : nlr * pq * 2.0 * nu * HU;
+                    pl.flops[b] += fl;
                     for (int sl = 0; sl < ns; sl++) {
                         const int a0 = (int)((int64_t)npu * sl / ns);
                         const int a1 = (int)((int64_t)npu * (sl + 1) / ns);
-                        ShellItem it{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
-                                     pl.npart + (int64_t)sl * nu * nlr};
-                        tmp[b].push_back({cr * (a1 - a0) / npu, it});
-                        // algorithmic flops: Boys and R once per primitive
-                        // quartet; per lane component the X contraction, then
-                        // either nu direct dot products per primitive quartet
-                        // or the sum over lane primitives per primitive
-                        // quartet plus nu dot products per uniform primitive
-                        const double pq = (double)(a1 - a0) * npl;
-                        pl.units[b] += pq * nlr;
-                        double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
-                        if (red)
-                            fl += nlr * (pq * HU + (a1 - a0) * 2.0 * nu * HU);
-                        else
-                            fl += nlr * pq * 2.0 * nu * HU;
-                        pl.flops[b] += fl;
+                        tmp[b].push_back(ShellItem{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
+                                                   pl.npart + (int64_t)sl * nu * nlr});
+                        tcost[b].push_back((float)(cr * (a1 - a0) / npu));
                     }
                     pl.npart += (int64_t)ns * nu * nlr;
                 }
             }
         }
     }
-    pl.items.clear();
-    for (int k = 0; k < 100; k++) {
-        pl.lo[k] = (int)pl.items.size();
-        std::stable_sort(tmp[k].begin(), tmp[k].end(),
-                         [](const std::pair<double, ShellItem> &a,
-                            const std::pair<double, ShellItem> &b) { return a.first > b.first; });
-        for (const auto &e : tmp[k]) pl.items.push_back(e.second);
+    // each bucket in descending cost (quarter-octave bins, counting sort), so the
+    // long items start first and the queue ends on short ones
+    size_t nitems = 0;
+    for (int b = 0; b < 100; b++) nitems += tmp[b].size();
+    pl.items.resize(nitems);
+    constexpr int NBIN = 256;
+    std::vector<uint8_t> bin;
+    size_t at = 0;
+    for (int b = 0; b < 100; b++) {
+        pl.lo[b] = (int)at;
+        const size_t n = tmp[b].size();
+        bin.resize(n);
+        int cnt[NBIN + 1] = {};
+        static_assert(NBIN <= 256, "bins are stored as bytes");
+        for (size_t t = 0; t < n; t++) {
+            const int e = (int)std::floor(4.0 * std::log2(std::max(tcost[b][t], 1.0f)));
+            bin[t] = (uint8_t)(NBIN - 1 - std::min(e, NBIN - 1)); // large cost -> small bin
+            cnt[bin[t] + 1]++;
+        }
+        for (int k = 0; k < NBIN; k++) cnt[k + 1] += cnt[k];
+        for (size_t t = 0; t < n; t++) pl.items[at + cnt[bin[t]]++] = tmp[b][t];
+        at += n;
     }
-    pl.lo[100] = (int)pl.items.size();
+    pl.lo[100] = (int)at;
 }
 
 } // namespace
