@@ -135,26 +135,92 @@ def roofline_eri(res):
             "all_eri_kernels_tflops": total_flops / (total_ms / 1e3) / 1e12}
 
 
-def roofline_estimator(res, nint, prec):
-    """The tiled Monte Carlo estimator: 13 algorithmic flops per
-    quartet-sample (3 differences, r^2, the two density products, the
-    accumulation) plus one reciprocal square root, over the estimator
-    kernel time measured with HIP events.  One launch per block."""
+def shell_units(bas):
+    """The library's shells (consecutive AOs with equal centre, fit, type and
+    order, at most 15) and its split of every shell pair's components into
+    units of 6, 3 and 1 (csrc/k_mc.hip, group_ready): returns, per shell pair
+    (I, K), the first shell I and the list of units as arrays of pair
+    indices."""
+    n = bas.n
+    lao = bas.npower.sum(1) + 2 * (bas.i_type == 2)
+    first = []
+    for i in range(n):
+        same = (i > 0 and i - first[-1] < 15 and bas.i_type[i] == bas.i_type[i - 1]
+                and bas.n_gauss[i] == bas.n_gauss[i - 1] and lao[i] == lao[i - 1]
+                and np.array_equal(bas.center[i], bas.center[i - 1])
+                and np.array_equal(bas.g_gauss[i], bas.g_gauss[i - 1])
+                and np.array_equal(bas.c_gauss[:, i], bas.c_gauss[:, i - 1]))
+        if not same:
+            first.append(i)
+    first.append(n)
+    out = []
+    for I in range(len(first) - 1):
+        for K in range(I + 1):
+            comps = [i * (i + 1) // 2 + k for i in range(first[I], first[I + 1])
+                     for k in range(first[K], first[K + 1]) if k <= i]
+            units, at = [], 0
+            for size in (6, 3, 1):
+                while len(comps) - at >= size:
+                    units.append(np.array(comps[at:at + size]))
+                    at += size
+            out.append((I, units))
+    return out
+
+
+def kernel_unit_pairs(bas):
+    """Unit pairs (row unit U, column unit V) holding at least one quartet
+    with row pair P >= column pair Q: the 1/r12 evaluations per sample."""
+    sps = shell_units(bas)
+    nsh = max(I for I, _ in sps) + 1
+    per_I = np.zeros(nsh)
+    for I, units in sps:
+        per_I[I] += len(units)
+    below = np.cumsum(per_I) - per_I           # units with first shell < I
+    total = float(np.sum(per_I * below))
+    by_I = {}
+    for I, units in sps:
+        by_I.setdefault(I, []).extend(units)
+    for units in by_I.values():                # same first shell: per unit pair
+        hi = np.array([u.max() for u in units])
+        lo = np.array([u.min() for u in units])
+        total += float(np.sum(hi[:, None] >= lo[None, :]))
+    return total
+
+
+def roofline_estimator(res, bas, prec):
+    """The group estimator of one block (9 class-pair kernels running side
+    by side, timed together with HIP events): algorithmic flops = 4 per
+    quartet-sample (two FMAs: K (A1 A2 + B1 B2)) + 8 per unit-pair-sample
+    (the three differences and r^2 of the kernel), and one reciprocal square
+    root per unit-pair-sample."""
     ms, n = res["mc_timings"]["estimator"]
-    qs = float(nint) * NBL * res["production_npts"]
-    achieved = 13.0 * qs / (ms / 1e3) / 1e12
+    samples = NBL * res["production_npts"]
+    nup = kernel_unit_pairs(bas)
+    flops = (4.0 * bas.nint + 8.0 * nup) * samples
+    achieved = flops / (ms / 1e3) / 1e12
     peak = PEAK_FP64 if prec != "fp32" else PEAK_FP32
-    name = "k_est_tile<double>" if prec != "fp32" else "k_est_tile<float>"
-    return {"bound": "fp64-valu" if prec != "fp32" else "fp32-valu", "kernel": name,
+    t = "double" if prec != "fp32" else "float"
+    traffic = None
+    tf = ROOT / "profiles" / "kernel_traffic.json"
+    if tf.exists():
+        try:
+            tj = json.loads(tf.read_text())
+            parts = [v for k, v in tj.items() if k.startswith(f"k_est_group<{t}")]
+            traffic = sum(parts) if parts else None
+        except Exception:
+            traffic = None
+    return {"bound": "fp64-valu" if prec != "fp32" else "fp32-valu",
+            "kernel": f"k_est_group<{t}, NR, NC, TR, TC> (9 class-pair launches per block)",
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": _traffic(name), "launches": n, "total_ms": ms,
-            "avg_launch_ms": ms / max(1, n), "flops_per_launch": 13.0 * qs / max(1, n),
-            "quartet_samples_per_s": qs / (ms / 1e3)}
+            "traffic": traffic, "launches": n, "total_ms": ms,
+            "avg_launch_ms": ms / max(1, n), "flops_per_launch": flops / max(1, n),
+            "quartet_samples_per_s": float(bas.nint) * samples / (ms / 1e3),
+            "kernel_evaluations_per_s": nup * samples / (ms / 1e3)}
 
 
-def roofline_of(res, nint):
+def roofline_of(res, bas):
     """The dominant kernel of the step (largest total kernel time)."""
-    eri, est = roofline_eri(res), roofline_estimator(res, nint, "fp64")
+    eri, est = roofline_eri(res), roofline_estimator(res, bas, "fp64")
     return (est, eri) if est["total_ms"] >= eri["total_ms"] else (eri, est)
 
 
@@ -248,7 +314,7 @@ def main():
         print(f"[bench] fp32 step: {res32['total']:.2f} s (MC {res32['mc']:.2f} s)",
               file=sys.stderr, flush=True)
     last = res64[-1]
-    roof, roof_other = roofline_of(last, bas.nint)
+    roof, roof_other = roofline_of(last, bas)
     precision = None
     if res32 is not None:
         # step 1 (seed 1) and the FP32 job share the pilot samples

@@ -14,6 +14,7 @@
 // Quartet accumulators are kept in pair-triangle order t = P(P+1)/2 + Q,
 // P >= Q (P = electron-1 pair), which is the reference's set of canonical
 // quartets; results are scattered to the reference kcp order at the end.
+#include <algorithm>
 #include <cmath>
 
 #include "zv_math.h"
@@ -24,7 +25,6 @@ namespace {
 constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
 constexpr double kPi0Factor = 1.0 / ((kTwoPi * kTwoPi) * kTwoPi);
 constexpr int kParityFields = 12;
-constexpr int kSplitFields = 10;
 
 // ------------------------------------------------------------------ Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -204,41 +204,6 @@ __global__ void k_dens_parity(Basis B, PairTab T, int np, int npp, int Mv, int M
     out[11 * plane + at] = s.jac2;
 }
 
-// Split form for the tiled estimator: S - G = K (Delta_1 rho_2 + rho^G_1 Delta_2)
-// with K = J_1 J_2 / (pi_0 r12) and Delta = rho - rho^G.  Fields [Mc][npp]:
-//   electron 1: u1x u1y u1z A1 = (J1/pi0) Delta1, B1 = (J1/pi0) rhoG1
-//   electron 2: u2x u2y u2z A2 = J2 rho2,        B2 = J2 Delta2
-// Padding pairs/samples get zero weights and points 2000 bohr apart.
-template <typename T>
-__global__ void k_dens_split(Basis B, PairTab Tb, int np, int npp, int Mv, int Mc,
-                             const double *__restrict__ r1, const double *__restrict__ r2,
-                             const double *__restrict__ pi0, T *__restrict__ out)
-{
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (int64_t)Mc * npp) return;
-    int p = (int)(gid % npp), m = (int)(gid / npp);
-    size_t plane = (size_t)Mc * npp, at = (size_t)m * npp + p;
-    T v[kSplitFields];
-    if (p >= np || m >= Mv) {
-        v[0] = (T)1000.0; v[1] = v[2] = (T)0.0; v[3] = v[4] = (T)0.0;
-        v[5] = (T)-1000.0; v[6] = v[7] = (T)0.0; v[8] = v[9] = (T)0.0;
-    } else {
-        double G[3] = {Tb.G[p], Tb.G[npp + p], Tb.G[2 * npp + p]};
-        PairSample s;
-        pair_sample(B, Tb.pi[p] - 1, Tb.pk[p] - 1, Tb.f[p], Tb.jf[p], G, r1 + 3 * m,
-                    r2 + 3 * m, s);
-        double c1 = s.jac1 / pi0[m];
-        v[0] = (T)s.u1[0]; v[1] = (T)s.u1[1]; v[2] = (T)s.u1[2];
-        v[3] = (T)(c1 * (s.rho1 - s.rhog1));
-        v[4] = (T)(c1 * s.rhog1);
-        v[5] = (T)s.u2[0]; v[6] = (T)s.u2[1]; v[7] = (T)s.u2[2];
-        v[8] = (T)(s.jac2 * s.rho2);
-        v[9] = (T)(s.jac2 * (s.rho2 - s.rhog2));
-    }
-#pragma unroll
-    for (int f = 0; f < kSplitFields; f++) out[f * plane + at] = v[f];
-}
-
 // ------------------------------------------------------------------ estimator
 __device__ inline void tri_decode(int64_t t, int64_t &P, int64_t &Q)
 {
@@ -289,10 +254,6 @@ __global__ void k_est_parity(const double *__restrict__ pd, const double *__rest
     eG[t] = s_G;
 }
 
-template <typename T> struct TileCfg;
-template <> struct TileCfg<float> { static constexpr int SUB = 16; };
-template <> struct TileCfg<double> { static constexpr int SUB = 8; };
-
 // FP32: the hardware reciprocal square root (about 1 ulp).  FP64: the
 // hardware estimate (about 5e-8 relative) and one Newton step,
 // y = y0 + y0 (1 - x y0^2) / 2, about 5e-15 relative (measured over 1e-9..1e9
@@ -307,80 +268,276 @@ __device__ inline double rsq(double x)
     return fma(y * 0.5, e, y);
 }
 
-// Tiled estimator: a workgroup owns a 64x64 tile of (P, Q) pairs with
-// P-tile >= Q-tile; each thread a 4x4 register tile.  Samples stream through
-// LDS in stages of SUB; per quartet-sample: 3 sub, r^2 (mul + 2 fma), one
-// rsqrt, t = A1 A2 + B1 B2 (mul + fma), acc += t rinv (fma).  Stage sums in
-// T, flushed to FP64 accumulators after each stage.
+// ------------------------------------------------------------------ group estimator
+// All AO pairs of one shell pair (the Cartesian components of two STO
+// shells) share f_P, G_P and the Jacobian, so u_P(r) and the 1/r12 kernel
+// of a quartet depend on the two shell pairs only (src/two_e_slater.F90:
+// 406-411,437-446).  In the split form
+//   S - G = sum_m K(m) [A1_P A2_Q + B1_P B2_Q],  K = 1/|u1_P - u2_Q|,
+// the weights fold J/pi0 and J into the component fields, so K is needed
+// once per (shell pair, shell pair, sample) and each component quartet costs
+// a multiply and two FMAs.
+//
+// Component fields live in shell-pair order (position pos = the shell
+// pair's first position + component, pair index comp[pos]), planes
+// [4][Mc][npos]: A1 = (J1/pi0)(rho1 - rhoG1), B1 = (J1/pi0) rhoG1,
+// A2 = J2 rho2, B2 = J2 (rho2 - rhoG2); the points per shell pair in
+// [6][Mc][nsp_pad]: u1 xyz, u2 xyz.  Samples m >= Mv get zero weights.
 template <typename T>
-__global__ __launch_bounds__(256) void k_est_tile(const T *__restrict__ e, int Mround, int Mc,
-                                                  int npp, int np, double *__restrict__ acc)
+__global__ void k_dens_group(Basis B, PairTab Tb, int npp, const int *__restrict__ pos_pair,
+                             const int *__restrict__ pos_sp, const int *__restrict__ sp_first,
+                             int npos, int nsp_pad, int Mv, int Mc, const double *__restrict__ r1,
+                             const double *__restrict__ r2, const double *__restrict__ pi0,
+                             T *__restrict__ comp, T *__restrict__ grp)
 {
-    constexpr int SUB = TileCfg<T>::SUB;
-    __shared__ T sP[5][SUB][ZV_TILE];
-    __shared__ T sQ[5][SUB][ZV_TILE];
-    int64_t tt = blockIdx.x;
-    int64_t bp64, bq64;
-    tri_decode(tt, bp64, bq64);
-    int bp = (int)bp64, bq = (int)bq64;
-    int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    size_t plane = (size_t)Mc * npp;
-    double acc64[4][4];
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)Mc * npos) return;
+    const int pos = (int)(gid % npos), m = (int)(gid / npos);
+    T v[4] = {(T)0, (T)0, (T)0, (T)0};
+    T u[6] = {(T)1000.0, (T)0, (T)0, (T)-1000.0, (T)0, (T)0};
+    if (m < Mv) {
+        const int p = pos_pair[pos];
+        double G[3] = {Tb.G[p], Tb.G[npp + p], Tb.G[2 * npp + p]};
+        PairSample sm;
+        pair_sample(B, Tb.pi[p] - 1, Tb.pk[p] - 1, Tb.f[p], Tb.jf[p], G, r1 + 3 * m, r2 + 3 * m,
+                    sm);
+        const double c1 = sm.jac1 / pi0[m];
+        v[0] = (T)(c1 * (sm.rho1 - sm.rhog1));
+        v[1] = (T)(c1 * sm.rhog1);
+        v[2] = (T)(sm.jac2 * sm.rho2);
+        v[3] = (T)(sm.jac2 * (sm.rho2 - sm.rhog2));
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc64[i][j] = 0.0;
-
-    for (int m0 = 0; m0 < Mround; m0 += SUB) {
-        for (int idx = threadIdx.x; idx < 5 * SUB * ZV_TILE; idx += 256) {
-            int f = idx / (SUB * ZV_TILE), rem = idx % (SUB * ZV_TILE);
-            int m = rem / ZV_TILE, q = rem % ZV_TILE;
-            size_t row = (size_t)(m0 + m) * npp;
-            sP[f][m][q] = e[f * plane + row + (size_t)bp * ZV_TILE + q];
-            sQ[f][m][q] = e[(5 + f) * plane + row + (size_t)bq * ZV_TILE + q];
+        for (int d = 0; d < 3; d++) {
+            u[d] = (T)sm.u1[d];
+            u[3 + d] = (T)sm.u2[d];
         }
-        __syncthreads();
-        T a[4][4];
+    }
+    const size_t cplane = (size_t)Mc * npos, at = (size_t)m * npos + pos;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+    for (int f = 0; f < 4; f++) comp[f * cplane + at] = v[f];
+    const int sp = pos_sp[pos];
+    if (sp_first[sp] == pos) {
+        const size_t gplane = (size_t)Mc * nsp_pad, ga = (size_t)m * nsp_pad + sp;
 #pragma unroll
-            for (int j = 0; j < 4; j++) a[i][j] = (T)0;
-        for (int m = 0; m < SUB; m++) {
-            T px[4], py[4], pz[4], pa[4], pb[4], qx[4], qy[4], qz[4], qa[4], qb[4];
+        for (int f = 0; f < 6; f++) grp[f * gplane + ga] = u[f];
+    }
+}
+
+// samples per LDS stage of the group estimator (deeper stages cost more LDS
+// than they gain in latency hiding: measured on MI355X)
+template <typename T> struct GroupSub;
+template <> struct GroupSub<float> { static constexpr int SUB = 16; };
+template <> struct GroupSub<double> { static constexpr int SUB = 8; };
+
+// A workgroup owns a tile of 16 TR row units x 16 TC column units (units of
+// NR resp. NC components of one shell pair); thread (tx, ty) owns TR row
+// units and TC column units: TR*TC kernel values and (TR NR) x (TC NC)
+// component quartets per sample.  Rows carry electron 1, columns electron
+// 2; an element is written only when its row pair P >= its column pair Q,
+// the reference's assignment (the electron-1 pair is the larger one), so
+// every quartet is written once.  Samples stream through LDS in stages of
+// SUB; stage sums in T are flushed to FP64.
+template <typename T, int NR, int NC, int TR, int TC>
+__global__ __launch_bounds__(256) void k_est_group(
+    const T *__restrict__ comp, const T *__restrict__ grp, const int *__restrict__ urow_pos,
+    const int *__restrict__ urow_sp, int nurow, const int *__restrict__ ucol_pos,
+    const int *__restrict__ ucol_sp, int nucol, const int2 *__restrict__ tiles,
+    const int *__restrict__ pos_pair, int Mround, int Mc, int npos, int nsp_pad,
+    double *__restrict__ acc)
+{
+    constexpr int SUB = GroupSub<T>::SUB;
+    constexpr int RU = 16 * TR, CU = 16 * TC, ROWS = RU * NR, COLS = CU * NC;
+    constexpr int ER = TR * NR, EC = TC * NC; // elements per thread
+    __shared__ T sA1[SUB][ROWS], sB1[SUB][ROWS], sA2[SUB][COLS], sB2[SUB][COLS];
+    __shared__ T sU1[3][SUB][RU], sU2[3][SUB][CU];
+    const int2 tl = tiles[blockIdx.x];
+    const int ru0 = tl.x * RU, cu0 = tl.y * CU;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const size_t cplane = (size_t)Mc * npos, gplane = (size_t)Mc * nsp_pad;
+    double acc64[ER][EC];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                px[i] = sP[0][m][ty * 4 + i]; py[i] = sP[1][m][ty * 4 + i];
-                pz[i] = sP[2][m][ty * 4 + i]; pa[i] = sP[3][m][ty * 4 + i];
-                pb[i] = sP[4][m][ty * 4 + i];
-                qx[i] = sQ[0][m][tx * 4 + i]; qy[i] = sQ[1][m][tx * 4 + i];
-                qz[i] = sQ[2][m][tx * 4 + i]; qa[i] = sQ[3][m][tx * 4 + i];
-                qb[i] = sQ[4][m][tx * 4 + i];
+    for (int i = 0; i < ER; i++)
+#pragma unroll
+        for (int j = 0; j < EC; j++) acc64[i][j] = 0.0;
+
+    // Load slots of this thread (fixed for every stage): source offset inside
+    // a stage of the component / point planes, -1 for padding.  The next
+    // stage's values are loaded into registers while the current one is
+    // computed, then stored to LDS behind the barrier.
+    constexpr int LA = (SUB * ROWS + 255) / 256, LB = (SUB * COLS + 255) / 256;
+    constexpr int LU = (3 * SUB * RU + 255) / 256, LV = (3 * SUB * CU + 255) / 256;
+    const int tid = threadIdx.x;
+    // offsets inside one stage fit 32 bits: SUB * npos and SUB * nsp_pad;
+    // the plane of a point field is added at load time
+    int sa[LA], sb[LB], su[LU], sv[LV];
+#pragma unroll
+    for (int k = 0; k < LA; k++) {
+        const int idx = tid + 256 * k, m = idx / ROWS, r = idx % ROWS, gu = ru0 + r / NR;
+        sa[k] = (idx < SUB * ROWS && gu < nurow) ? m * npos + urow_pos[gu] + r % NR : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < LB; k++) {
+        const int idx = tid + 256 * k, m = idx / COLS, r = idx % COLS, gu = cu0 + r / NC;
+        sb[k] = (idx < SUB * COLS && gu < nucol) ? m * npos + ucol_pos[gu] + r % NC : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < LU; k++) {
+        const int idx = tid + 256 * k, rem = idx % (SUB * RU);
+        const int m = rem / RU, gu = ru0 + rem % RU;
+        su[k] = (idx < 3 * SUB * RU && gu < nurow) ? m * nsp_pad + urow_sp[gu] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < LV; k++) {
+        const int idx = tid + 256 * k, rem = idx % (SUB * CU);
+        const int m = rem / CU, gu = cu0 + rem % CU;
+        sv[k] = (idx < 3 * SUB * CU && gu < nucol) ? m * nsp_pad + ucol_sp[gu] : -1;
+    }
+    T va[LA], vb[LA], vc[LB], vd[LB], vu[LU], vv[LV];
+    auto gload = [&](int m0) {
+        const T *c0 = comp + (int64_t)m0 * npos, *g0 = grp + (int64_t)m0 * nsp_pad;
+#pragma unroll
+        for (int k = 0; k < LA; k++) {
+            va[k] = sa[k] >= 0 ? c0[sa[k]] : (T)0;
+            vb[k] = sa[k] >= 0 ? c0[cplane + sa[k]] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < LB; k++) {
+            vc[k] = sb[k] >= 0 ? c0[2 * cplane + sb[k]] : (T)0;
+            vd[k] = sb[k] >= 0 ? c0[3 * cplane + sb[k]] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < LU; k++) {
+            const int f = (tid + 256 * k) / (SUB * RU);
+            vu[k] = su[k] >= 0 ? g0[f * gplane + su[k]] : (f == 0 ? (T)1000.0 : (T)0);
+        }
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const int f = (tid + 256 * k) / (SUB * CU);
+            vv[k] = sv[k] >= 0 ? g0[(3 + f) * gplane + sv[k]] : (f == 0 ? (T)-1000.0 : (T)0);
+        }
+    };
+    auto lstore = [&]() {
+        T *fA1 = &sA1[0][0], *fB1 = &sB1[0][0], *fA2 = &sA2[0][0], *fB2 = &sB2[0][0];
+        T *fU1 = &sU1[0][0][0], *fU2 = &sU2[0][0][0];
+#pragma unroll
+        for (int k = 0; k < LA; k++)
+            if (tid + 256 * k < SUB * ROWS) {
+                fA1[tid + 256 * k] = va[k];
+                fB1[tid + 256 * k] = vb[k];
             }
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+        for (int k = 0; k < LB; k++)
+            if (tid + 256 * k < SUB * COLS) {
+                fA2[tid + 256 * k] = vc[k];
+                fB2[tid + 256 * k] = vd[k];
+            }
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    T dx = px[i] - qx[j], dy = py[i] - qy[j], dz = pz[i] - qz[j];
-                    T r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    T w = fma(pb[i], qb[j], pa[i] * qa[j]);
-                    a[i][j] = fma(w, rsq(r2), a[i][j]);
+        for (int k = 0; k < LU; k++)
+            if (tid + 256 * k < 3 * SUB * RU) fU1[tid + 256 * k] = vu[k];
+#pragma unroll
+        for (int k = 0; k < LV; k++)
+            if (tid + 256 * k < 3 * SUB * CU) fU2[tid + 256 * k] = vv[k];
+    };
+    gload(0);
+    for (int m0 = 0; m0 < Mround; m0 += SUB) {
+        lstore();
+        __syncthreads();
+        if (m0 + SUB < Mround) gload(m0 + SUB);
+        // FP32 sums a stage in T and flushes it to FP64; FP64 accumulates
+        // directly.
+        T a[ER][EC];
+#pragma unroll
+        for (int i = 0; i < ER; i++)
+#pragma unroll
+            for (int j = 0; j < EC; j++) a[i][j] = (T)0;
+        T u1[TR][3], u2[TC][3], a1[ER], b1[ER], a2[EC], b2[EC];
+        auto fetch = [&](int m) {
+#pragma unroll
+            for (int i = 0; i < TR; i++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) u1[i][d] = sU1[d][m][ty * TR + i];
+#pragma unroll
+            for (int j = 0; j < TC; j++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) u2[j][d] = sU2[d][m][tx * TC + j];
+#pragma unroll
+            for (int i = 0; i < ER; i++) {
+                a1[i] = sA1[m][ty * ER + i];
+                b1[i] = sB1[m][ty * ER + i];
+            }
+#pragma unroll
+            for (int j = 0; j < EC; j++) {
+                a2[j] = sA2[m][tx * EC + j];
+                b2[j] = sB2[m][tx * EC + j];
+            }
+        };
+#pragma unroll
+        for (int m = 0; m < SUB; m++) {
+            fetch(m);
+            T K[TR][TC];
+#pragma unroll
+            for (int i = 0; i < TR; i++)
+#pragma unroll
+                for (int j = 0; j < TC; j++) {
+                    const T dx = u1[i][0] - u2[j][0], dy = u1[i][1] - u2[j][1],
+                            dz = u1[i][2] - u2[j][2];
+                    K[i][j] = rsq(fma(dz, dz, fma(dy, dy, dx * dx)));
                 }
+            T ka[ER][TC], kb[ER][TC], c2[EC], d2[EC];
+#pragma unroll
+            for (int i = 0; i < ER; i++)
+#pragma unroll
+                for (int j = 0; j < TC; j++) {
+                    ka[i][j] = K[i / NR][j] * a1[i];
+                    kb[i][j] = K[i / NR][j] * b1[i];
+                }
+#pragma unroll
+            for (int j = 0; j < EC; j++) {
+                c2[j] = a2[j];
+                d2[j] = b2[j];
+            }
+#pragma unroll
+            for (int i = 0; i < ER; i++)
+#pragma unroll
+                for (int j = 0; j < TC; j++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) {
+                        if constexpr (sizeof(T) == 8)
+                            acc64[i][j * NC + c] =
+                                fma(ka[i][j], c2[j * NC + c],
+                                    fma(kb[i][j], d2[j * NC + c], acc64[i][j * NC + c]));
+                        else
+                            a[i][j * NC + c] = fma(ka[i][j], c2[j * NC + c],
+                                                   fma(kb[i][j], d2[j * NC + c], a[i][j * NC + c]));
+                    }
         }
+        if constexpr (sizeof(T) != 8) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+            for (int i = 0; i < ER; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc64[i][j] += (double)a[i][j];
+                for (int j = 0; j < EC; j++) acc64[i][j] += (double)a[i][j];
+        }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int P = bp * ZV_TILE + ty * 4 + i;
+    for (int i = 0; i < TR; i++) {
+        const int gu = ru0 + ty * TR + i;
+        if (gu >= nurow) continue;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            int Q = bq * ZV_TILE + tx * 4 + j;
-            if (P < np && Q <= P) {
-                size_t t = (size_t)zv_tri(P, Q);
-                acc[t] = acc[t] + acc64[i][j];
+        for (int r = 0; r < NR; r++) {
+            const int P = pos_pair[urow_pos[gu] + r];
+#pragma unroll
+            for (int j = 0; j < TC; j++) {
+                const int gv = cu0 + tx * TC + j;
+                if (gv >= nucol) continue;
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int Q = pos_pair[ucol_pos[gv] + c];
+                    if (P >= Q) {
+                        const size_t t = (size_t)zv_tri(P, Q);
+                        acc[t] = acc[t] + acc64[i * NR + r][j * NC + c];
+                    }
+                }
             }
         }
     }
@@ -546,17 +703,164 @@ int zv_run_pair_samples(zv_context *c, int i, int k, int ns, const double *r1, c
 // Chunk size (samples) for a per-chunk buffer budget.
 static int pick_chunk(zv_context *c, int64_t npts, int precision)
 {
-    size_t per = (precision == ZV_PREC_FP64_PARITY) ? kParityFields * 8
-                 : (precision == ZV_PREC_FP64)      ? kSplitFields * 8
-                                                    : kSplitFields * 4;
+    // bytes per sample of per-(pair, sample) data
+    size_t per = (precision == ZV_PREC_FP64_PARITY)
+                     ? (size_t)kParityFields * 8 * c->npair_pad
+                     : (size_t)(precision == ZV_PREC_FP64 ? 8 : 4) *
+                           (4 * (size_t)c->npair + 6 * (size_t)c->nsp);
     size_t budget = (size_t)2 << 30; // 2 GiB of per-(pair, sample) data
-    int64_t mc = (int64_t)(budget / (per * (size_t)c->npair_pad));
-    mc = mc / 80 * 80;
-    if (mc < 80) mc = 80;
-    if (mc > 65520) mc = 65520;
-    int64_t need = (npts + 79) / 80 * 80;
+    // a multiple of 160: of the 40-walker blocks of the parity estimator and
+    // of every LDS stage depth (8, 16, 32), so a rounded-up stage never
+    // reads past the chunk
+    int64_t mc = (int64_t)(budget / per);
+    mc = mc / 160 * 160;
+    if (mc < 160) mc = 160;
+    if (mc > 65440) mc = 65440;
+    int64_t need = (npts + 159) / 160 * 160;
     if (mc > need) mc = need;
     return (int)mc;
+}
+
+// ---- group estimator tables -------------------------------------------------
+// unit classes: 1, 3 or 6 components of one shell pair
+constexpr int kUnitRows[3] = {1, 3, 6};
+// units per thread (TR row units, TC column units) for each pair of classes
+constexpr int kUnitsPerThread[3][3][2] = {{{4, 4}, {4, 2}, {4, 1}},
+                                          {{2, 4}, {2, 2}, {2, 1}},
+                                          {{1, 4}, {1, 2}, {1, 1}}};
+
+// Units (greedy split of each shell pair's components into 6, 3 and 1),
+// ordered by the first shell of their shell pair, and the tiles of every
+// pair of classes that hold at least one element with P >= Q: a row unit
+// of shell pair (I, K) and a column unit of (J, L) can hold P >= Q only
+// when I >= J.
+static int group_ready(zv_context *c)
+{
+    if (c->grp_ready) return ZV_OK;
+    const int nsp = c->nsp, npair = c->npair;
+    std::vector<int> sp_I(nsp);
+    for (int I = 0; I < c->nshell; I++)
+        for (int K = 0; K <= I; K++) sp_I[I * (I + 1) / 2 + K] = I;
+    std::vector<int> upos[3], usp[3], uI[3];
+    for (int sp = 0; sp < nsp; sp++) {
+        int at = c->h_sp_coff[sp];
+        int rem = c->h_sp_coff[sp + 1] - at;
+        for (int k = 2; k >= 0; k--)
+            while (rem >= kUnitRows[k]) {
+                upos[k].push_back(at);
+                usp[k].push_back(sp);
+                uI[k].push_back(sp_I[sp]);
+                at += kUnitRows[k];
+                rem -= kUnitRows[k];
+            }
+    }
+    std::vector<int> &g = c->h_gmeta;
+    g.clear();
+    g.insert(g.end(), c->h_sp_comp.begin(), c->h_sp_comp.end()); // pos -> pair
+    c->grp_possp_off = (int64_t)g.size();
+    std::vector<int> pos_sp(npair);
+    for (int sp = 0; sp < nsp; sp++)
+        for (int t = c->h_sp_coff[sp]; t < c->h_sp_coff[sp + 1]; t++) pos_sp[t] = sp;
+    g.insert(g.end(), pos_sp.begin(), pos_sp.end());
+    g.insert(g.end(), c->h_sp_coff.begin(), c->h_sp_coff.end()); // sp -> first position
+    for (int k = 0; k < 3; k++) {
+        c->grp_nunit[k] = (int)upos[k].size();
+        c->grp_unit_off[k] = (int64_t)g.size();
+        g.insert(g.end(), upos[k].begin(), upos[k].end());
+        g.insert(g.end(), usp[k].begin(), usp[k].end());
+    }
+    if (g.size() & 1) g.push_back(0); // int2 alignment of the tiles
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            const int RU = 16 * kUnitsPerThread[a][b][0], CU = 16 * kUnitsPerThread[a][b][1];
+            const int nr = c->grp_nunit[a], ncl = c->grp_nunit[b];
+            const int nrt = (nr + RU - 1) / RU, nct = (ncl + CU - 1) / CU;
+            c->grp_tile_off[3 * a + b] = (int64_t)g.size() / 2;
+            int cnt = 0;
+            for (int rt = 0; rt < nrt; rt++) {
+                const int maxI = uI[a][std::min(nr, (rt + 1) * RU) - 1];
+                for (int ct = 0; ct < nct; ct++) {
+                    const int minI = uI[b][ct * CU];
+                    if (maxI < minI) continue;
+                    g.push_back(rt);
+                    g.push_back(ct);
+                    cnt++;
+                }
+            }
+            c->grp_ntile[3 * a + b] = cnt;
+        }
+    c->grp_nsp_pad = nsp;
+    ZV_CHECK(zv_ensure(c, c->b_gmeta, sizeof(int) * g.size()));
+    ZV_HIPCHECK(c, hipMemcpyAsync(c->b_gmeta.p, g.data(), sizeof(int) * g.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->grp_ready = true;
+    return ZV_OK;
+}
+
+template <typename T, int NR, int NC, int TR, int TC>
+static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const T *comp,
+                           const T *grp, const int *urp, const int *urs, int nur, const int *ucp,
+                           const int *ucs, int nuc, const int2 *tiles, const int *pos_pair,
+                           int Mround, int Mc, double *acc)
+{
+    hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC>), dim3(ntile), dim3(256), 0, st, comp,
+                       grp, urp, urs, nur, ucp, ucs, nuc, tiles, pos_pair, Mround, Mc, c->npair,
+                       c->grp_nsp_pad, acc);
+}
+
+// One launch per pair of unit classes; each is too small to fill the device
+// alone, so they run side by side on the main and the side streams (they
+// write disjoint quartets) and are joined back into the main stream.
+template <typename T>
+static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int Mround, int Mc,
+                               double *acc)
+{
+    const int *g = (const int *)c->b_gmeta.p;
+    const int *pos_pair = g;
+    hipStream_t sts[1 + zv_context::NSIDE] = {c->stream};
+    for (int k = 0; k < zv_context::NSIDE; k++) sts[1 + k] = c->side[k];
+    ZV_HIPCHECK(c, hipEventRecord(c->ev_fork, c->stream));
+    for (int k = 0; k < zv_context::NSIDE; k++)
+        ZV_HIPCHECK(c, hipStreamWaitEvent(c->side[k], c->ev_fork, 0));
+    // largest launches first
+    int order[9];
+    for (int k = 0; k < 9; k++) order[k] = k;
+    std::sort(order, order + 9, [&](int x, int y) { return c->grp_ntile[x] > c->grp_ntile[y]; });
+    int nl = 0;
+    for (int oi = 0; oi < 9; oi++) {
+        const int a = order[oi] / 3, b = order[oi] % 3;
+        {
+            const unsigned nt = (unsigned)c->grp_ntile[3 * a + b];
+            if (!nt) continue;
+            hipStream_t st = sts[nl++ % (1 + zv_context::NSIDE)];
+            const int *urp = g + c->grp_unit_off[a], *urs = urp + c->grp_nunit[a];
+            const int *ucp = g + c->grp_unit_off[b], *ucs = ucp + c->grp_nunit[b];
+            const int nur = c->grp_nunit[a], nuc = c->grp_nunit[b];
+            const int2 *tiles = (const int2 *)g + c->grp_tile_off[3 * a + b];
+#define ZV_GROUP(NR, NC, TR, TC)                                                                  \
+    launch_group_t<T, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
+                                      pos_pair, Mround, Mc, acc)
+            switch (3 * a + b) {
+            case 0: ZV_GROUP(1, 1, 4, 4); break;
+            case 1: ZV_GROUP(1, 3, 4, 2); break;
+            case 2: ZV_GROUP(1, 6, 4, 1); break;
+            case 3: ZV_GROUP(3, 1, 2, 4); break;
+            case 4: ZV_GROUP(3, 3, 2, 2); break;
+            case 5: ZV_GROUP(3, 6, 2, 1); break;
+            case 6: ZV_GROUP(6, 1, 1, 4); break;
+            case 7: ZV_GROUP(6, 3, 1, 2); break;
+            case 8: ZV_GROUP(6, 6, 1, 1); break;
+            }
+#undef ZV_GROUP
+            ZV_HIPCHECK(c, hipGetLastError());
+        }
+    }
+    for (int k = 0; k < zv_context::NSIDE; k++) {
+        ZV_HIPCHECK(c, hipEventRecord(c->ev_join[k], c->side[k]));
+        ZV_HIPCHECK(c, hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
+    }
+    return ZV_OK;
 }
 
 int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uint64_t stream,
@@ -567,11 +871,14 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
     const int np = c->npair, npp = c->npair_pad;
     const int Mc = pick_chunk(c, npts, precision);
     size_t fbytes = precision == ZV_PREC_FP32 ? 4 : 8;
-    size_t nfield = precision == ZV_PREC_FP64_PARITY ? kParityFields : kSplitFields;
+    if (precision != ZV_PREC_FP64_PARITY) ZV_CHECK(group_ready(c));
+    const size_t ncomp = (size_t)4 * Mc * np, ngrp = (size_t)6 * Mc * c->nsp;
     ZV_CHECK(zv_ensure(c, c->b_r1, sizeof(double) * 3 * Mc));
     ZV_CHECK(zv_ensure(c, c->b_r2, sizeof(double) * 3 * Mc));
     ZV_CHECK(zv_ensure(c, c->b_pi0, sizeof(double) * Mc));
-    ZV_CHECK(zv_ensure(c, c->b_pd, fbytes * nfield * (size_t)Mc * npp));
+    ZV_CHECK(zv_ensure(c, c->b_pd, precision == ZV_PREC_FP64_PARITY
+                                       ? fbytes * kParityFields * (size_t)Mc * npp
+                                       : fbytes * (ncomp + ngrp)));
     ZV_CHECK(zv_ensure(c, c->b_accS, sizeof(double) * ntri));
     if (precision == ZV_PREC_FP64_PARITY) ZV_CHECK(zv_ensure(c, c->b_accG, sizeof(double) * ntri));
     ZV_CHECK(zv_ensure(c, c->b_s1, sizeof(double) * ntri));
@@ -594,7 +901,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
         for (int64_t m0 = 0; m0 < npts; m0 += Mc) {
             int Mv = (int)((npts - m0) < Mc ? (npts - m0) : Mc);
             uint64_t g0 = (uint64_t)b * (uint64_t)npts + (uint64_t)m0;
-            int sub = precision == ZV_PREC_FP32 ? TileCfg<float>::SUB : TileCfg<double>::SUB;
+            int sub = precision == ZV_PREC_FP32 ? GroupSub<float>::SUB : GroupSub<double>::SUB;
             int Mround = (Mv + sub - 1) / sub * sub;
             if (mode == ZV_SAMPLES_FIXED) {
                 PhaseTimer tm(c, PH_COPY);
@@ -624,33 +931,31 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                                    accG);
                 ZV_HIPCHECK(c, hipGetLastError());
             } else {
-                int nbt = npp / ZV_TILE;
-                unsigned ntiles = (unsigned)((int64_t)nbt * (nbt + 1) / 2);
+                const int *g = (const int *)c->b_gmeta.p;
+                const int *pos_pair = g, *pos_sp = g + c->grp_possp_off, *sp_first = pos_sp + np;
+                const unsigned nb = nblk((int64_t)Mc * np, 256);
                 if (precision == ZV_PREC_FP32) {
+                    float *comp = (float *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
-                        hipLaunchKernelGGL(k_dens_split<float>, dim3(nblk((int64_t)Mc * npp, 256)),
-                                           dim3(256), 0, c->stream, B, PT, np, npp, Mv, Mc, d_r1,
-                                           d_r2, d_pi0, (float *)c->b_pd.p);
+                        hipLaunchKernelGGL(k_dens_group<float>, dim3(nb), dim3(256), 0, c->stream,
+                                           B, PT, npp, pos_pair, pos_sp, sp_first, np,
+                                           c->grp_nsp_pad, Mv, Mc, d_r1, d_r2, d_pi0, comp, grp);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    hipLaunchKernelGGL(k_est_tile<float>, dim3(ntiles), dim3(256), 0, c->stream,
-                                       (const float *)c->b_pd.p, Mround, Mc, npp, np, accS);
-                    ZV_HIPCHECK(c, hipGetLastError());
+                    ZV_CHECK(run_group_estimator<float>(c, comp, grp, Mround, Mc, accS));
                 } else {
+                    double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
-                        hipLaunchKernelGGL(k_dens_split<double>,
-                                           dim3(nblk((int64_t)Mc * npp, 256)), dim3(256), 0,
-                                           c->stream, B, PT, np, npp, Mv, Mc, d_r1, d_r2, d_pi0,
-                                           (double *)c->b_pd.p);
+                        hipLaunchKernelGGL(k_dens_group<double>, dim3(nb), dim3(256), 0, c->stream,
+                                           B, PT, npp, pos_pair, pos_sp, sp_first, np,
+                                           c->grp_nsp_pad, Mv, Mc, d_r1, d_r2, d_pi0, comp, grp);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    hipLaunchKernelGGL(k_est_tile<double>, dim3(ntiles), dim3(256), 0, c->stream,
-                                       (const double *)c->b_pd.p, Mround, Mc, npp, np, accS);
-                    ZV_HIPCHECK(c, hipGetLastError());
+                    ZV_CHECK(run_group_estimator<double>(c, comp, grp, Mround, Mc, accS));
                 }
             }
         }

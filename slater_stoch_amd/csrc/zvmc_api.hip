@@ -174,8 +174,13 @@ extern "C" int zv_create(int device, zv_context **out)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ZV_ERR_NODEV;
     zv_context *c = new zv_context();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-        delete c;
+    bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&c->stream) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < zv_context::NSIDE; k++)
+        ok = hipStreamCreate(&c->side[k]) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        zv_destroy(c);
         return ZV_ERR_HIP;
     }
     *out = c;
@@ -197,6 +202,7 @@ static void free_basis(zv_context *c)
     c->d_kbase = nullptr;
     c->pp_ready = false;
     c->sp_ready = false;
+    c->grp_ready = false;
 }
 
 extern "C" int zv_destroy(zv_context *c)
@@ -210,11 +216,17 @@ extern "C" int zv_destroy(zv_context *c)
                       &c->b_s1, &c->b_s2, &c->b_out, &c->b_out2, &c->b_blk, &c->b_pp,
                       &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc,
                       &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
-                      &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist};
+                      &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist,
+                      &c->b_gmeta};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
+    for (int k = 0; k < zv_context::NSIDE; k++) {
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ZV_OK;
 }

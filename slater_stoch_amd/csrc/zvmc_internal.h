@@ -13,8 +13,7 @@
 
 #define ZV_GMAX 32     // max Gaussians per STO fit handled on the device
 #define ZV_NW 40       // walker block of the reference estimator (src/common_data.F90:4)
-#define ZV_TILE 64     // pair tile of the fast estimator
-#define ZV_SUB 32      // samples per LDS stage of the fast estimator
+#define ZV_TILE 64     // pair-table padding (the parity estimator's pair planes)
 #define ZV_LMAX_T 4    // largest pair angular momentum with a templated ERI kernel
 #define ZV_LPAIR_MAX 8 // largest pair angular momentum supported at all
 #define ZV_SHMAX 15    // largest shell (Cartesian components sharing one Gaussian fit)
@@ -32,6 +31,12 @@ struct zv_context {
     int device = 0;
     int verbose = 0;
     hipStream_t stream = nullptr;
+    // extra streams (default priority) for independent launches that are
+    // each too small to fill the device; forked from and joined back into
+    // `stream` inside one call
+    static constexpr int NSIDE = 4;
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
     std::string err;
 
     // basis (host)
@@ -79,11 +84,22 @@ struct zv_context {
     DevBuf b_misc;
     bool pp_ready = false;
     // shell-pair data: component tables and the uniform-side copies
-    DevBuf b_spmeta;                   // comp[npair], coff[nsp+1], puoff[nsp], euoff[nsp]
+    DevBuf b_spmeta;                   // puoff[nsp], euoff[nsp], comp[npair], coff[nsp+1]
     DevBuf b_spu, b_spe;               // per primitive pair: p,1/p,P,c; E [a][comp][h]
     std::vector<int> h_sp_np;          // kept primitive pairs per shell pair
     int64_t sp_eu_total = 0;           // doubles of one uniform-side E layout
     bool sp_ready = false;
+    // group-blocked estimator: units of 1, 3 or 6 components of one shell
+    // pair per class, tiles per pair of classes (built on first use per basis)
+    bool grp_ready = false;
+    DevBuf b_gmeta;
+    int grp_nunit[3] = {};
+    int64_t grp_unit_off[3] = {};     // int offsets in b_gmeta: pos, then sp, per class
+    int grp_ntile[9] = {};
+    int64_t grp_tile_off[9] = {};     // int2 offsets in b_gmeta
+    int64_t grp_possp_off = 0;
+    int grp_nsp_pad = 0;
+    std::vector<int> h_gmeta;
     // planned shell work of the two Schwarz phases (host copies kept alive for the copies)
     DevBuf b_sitem1, b_sgroup1, b_part1, b_sitem2, b_sgroup2, b_part2, b_plist;
     std::vector<char> h_plan1, h_plan2;
