@@ -280,46 +280,85 @@ __device__ inline double rsq(double x)
 //
 // Component fields live in shell-pair order (position pos = the shell
 // pair's first position + component, pair index comp[pos]), planes
-// [4][Mc][npos]: A1 = (J1/pi0)(rho1 - rhoG1), B1 = (J1/pi0) rhoG1,
-// A2 = J2 rho2, B2 = J2 (rho2 - rhoG2); the points per shell pair in
-// [6][Mc][nsp_pad]: u1 xyz, u2 xyz.  Samples m >= Mv get zero weights.
+// [4][npos][Mc] with the sample fastest: A1 = (J1/pi0)(rho1 - rhoG1),
+// B1 = (J1/pi0) rhoG1, A2 = J2 rho2, B2 = J2 (rho2 - rhoG2); the points per
+// shell pair in [6][nsp][Mc]: u1 xyz, u2 xyz.  Samples m >= Mv get zero
+// weights and points 2000 bohr apart.
+//
+// One thread per (shell pair, sample), the lanes of a wave on consecutive
+// samples of one shell pair: the transform, the Jacobians and the radial
+// parts (STO and 20-Gaussian fit, 84 exponentials) are computed once for
+// the shell pair and every component only multiplies its Cartesian
+// polynomials in, with the operations of pair_density in the same order.
 template <typename T>
 __global__ void k_dens_group(Basis B, PairTab Tb, int npp, const int *__restrict__ pos_pair,
-                             const int *__restrict__ pos_sp, const int *__restrict__ sp_first,
-                             int npos, int nsp_pad, int Mv, int Mc, const double *__restrict__ r1,
-                             const double *__restrict__ r2, const double *__restrict__ pi0,
-                             T *__restrict__ comp, T *__restrict__ grp)
+                             const int *__restrict__ sp_first, int npos, int nsp, int Mv, int Mc,
+                             const double *__restrict__ r1, const double *__restrict__ r2,
+                             const double *__restrict__ pi0, T *__restrict__ comp,
+                             T *__restrict__ grp)
 {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (int64_t)Mc * npos) return;
-    const int pos = (int)(gid % npos), m = (int)(gid / npos);
-    T v[4] = {(T)0, (T)0, (T)0, (T)0};
-    T u[6] = {(T)1000.0, (T)0, (T)0, (T)-1000.0, (T)0, (T)0};
-    if (m < Mv) {
-        const int p = pos_pair[pos];
-        double G[3] = {Tb.G[p], Tb.G[npp + p], Tb.G[2 * npp + p]};
-        PairSample sm;
-        pair_sample(B, Tb.pi[p] - 1, Tb.pk[p] - 1, Tb.f[p], Tb.jf[p], G, r1 + 3 * m, r2 + 3 * m,
-                    sm);
-        const double c1 = sm.jac1 / pi0[m];
-        v[0] = (T)(c1 * (sm.rho1 - sm.rhog1));
-        v[1] = (T)(c1 * sm.rhog1);
-        v[2] = (T)(sm.jac2 * sm.rho2);
-        v[3] = (T)(sm.jac2 * (sm.rho2 - sm.rhog2));
+    const int mblocks = (Mc + blockDim.x - 1) / blockDim.x;
+    const int sp = blockIdx.x / mblocks;
+    const int m = (blockIdx.x % mblocks) * blockDim.x + threadIdx.x;
+    if (m >= Mc) return;
+    const int pos0 = sp_first[sp], nc = sp_first[sp + 1] - pos0;
+    const size_t cplane = (size_t)npos * Mc, gplane = (size_t)nsp * Mc;
+    T *g = grp + (size_t)sp * Mc + m;
+    if (m >= Mv) {
+        const T u[6] = {(T)1000.0, (T)0, (T)0, (T)-1000.0, (T)0, (T)0};
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            u[d] = (T)sm.u1[d];
-            u[3 + d] = (T)sm.u2[d];
-        }
+        for (int f = 0; f < 6; f++) g[f * gplane] = u[f];
+        for (int t = 0; t < nc; t++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) comp[f * cplane + (size_t)(pos0 + t) * Mc + m] = (T)0;
+        return;
     }
-    const size_t cplane = (size_t)Mc * npos, at = (size_t)m * npos + pos;
+    // shell-pair quantities from the first component (src/two_e_slater.F90:406-414)
+    const int p0 = pos_pair[pos0];
+    const int oi0 = Tb.pi[p0] - 1, ok0 = Tb.pk[p0] - 1;
+    const double f = Tb.f[p0], jf = Tb.jf[p0];
+    const double *a = r1 + 3 * m, *b = r2 + 3 * m;
+    const double m1 = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+    const double m2 = (b[0] * b[0] + b[1] * b[1]) + b[2] * b[2];
+    const double r1m = sqrt(m1), r2m = sqrt(m2);
+    double u1[3], u2[3];
 #pragma unroll
-    for (int f = 0; f < 4; f++) comp[f * cplane + at] = v[f];
-    const int sp = pos_sp[pos];
-    if (sp_first[sp] == pos) {
-        const size_t gplane = (size_t)Mc * nsp_pad, ga = (size_t)m * nsp_pad + sp;
+    for (int d = 0; d < 3; d++) {
+        const double G = Tb.G[d * npp + p0];
+        u1[d] = (f * a[d]) * r1m + G;
+        u2[d] = (f * b[d]) * r2m + G;
+    }
+    const double f1 = jf * r1m, f2 = jf * r2m;
+    const double jac1 = 0.25 * fabs((f1 * f1) * f1), jac2 = 0.25 * fabs((f2 * f2) * f2);
+    const double c1 = jac1 / pi0[m];
+    // radial parts of the two shells at both points (src/u_orb.F90, src/u_gauss.F90)
+    double ri1, rk1, ri2, rk2;
+    (void)poly_at(B, oi0, u1, &ri1);
+    (void)poly_at(B, ok0, u1, &rk1);
+    (void)poly_at(B, oi0, u2, &ri2);
+    (void)poly_at(B, ok0, u2, &rk2);
+    const double oi1 = u_orb(B.nsto[oi0], B.zeta[oi0], ri1), ok1 = u_orb(B.nsto[ok0], B.zeta[ok0], rk1);
+    const double oi2 = u_orb(B.nsto[oi0], B.zeta[oi0], ri2), ok2 = u_orb(B.nsto[ok0], B.zeta[ok0], rk2);
+    const double gi1 = u_gauss(B, oi0, ri1), gk1 = u_gauss(B, ok0, rk1);
+    const double gi2 = u_gauss(B, oi0, ri2), gk2 = u_gauss(B, ok0, rk2);
 #pragma unroll
-        for (int f = 0; f < 6; f++) grp[f * gplane + ga] = u[f];
+    for (int d = 0; d < 3; d++) {
+        g[d * gplane] = (T)u1[d];
+        g[(3 + d) * gplane] = (T)u2[d];
+    }
+    for (int t = 0; t < nc; t++) {
+        const int p = pos_pair[pos0 + t];
+        const int oi = Tb.pi[p] - 1, ok = Tb.pk[p] - 1;
+        double rr;
+        const double pik1 = poly_at(B, oi, u1, &rr) * poly_at(B, ok, u1, &rr);
+        const double pik2 = poly_at(B, oi, u2, &rr) * poly_at(B, ok, u2, &rr);
+        const double rho1 = (pik1 * oi1) * ok1, rhog1 = (pik1 * gi1) * gk1;
+        const double rho2 = (pik2 * oi2) * ok2, rhog2 = (pik2 * gi2) * gk2;
+        T *o = comp + (size_t)(pos0 + t) * Mc + m;
+        o[0] = (T)(c1 * (rho1 - rhog1));
+        o[cplane] = (T)(c1 * rhog1);
+        o[2 * cplane] = (T)(jac2 * rho2);
+        o[3 * cplane] = (T)(jac2 * (rho2 - rhog2));
     }
 }
 
@@ -353,7 +392,7 @@ __global__ __launch_bounds__(256) void k_est_group(
     const int2 tl = tiles[blockIdx.x];
     const int ru0 = tl.x * RU, cu0 = tl.y * CU;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const size_t cplane = (size_t)Mc * npos, gplane = (size_t)Mc * nsp_pad;
+    const size_t cplane = (size_t)npos * Mc, gplane = (size_t)nsp_pad * Mc;
     double acc64[ER][EC];
 #pragma unroll
     for (int i = 0; i < ER; i++)
@@ -367,34 +406,37 @@ __global__ __launch_bounds__(256) void k_est_group(
     constexpr int LA = (SUB * ROWS + 255) / 256, LB = (SUB * COLS + 255) / 256;
     constexpr int LU = (3 * SUB * RU + 255) / 256, LV = (3 * SUB * CU + 255) / 256;
     const int tid = threadIdx.x;
-    // offsets inside one stage fit 32 bits: SUB * npos and SUB * nsp_pad;
-    // the plane of a point field is added at load time
-    int sa[LA], sb[LB], su[LU], sv[LV];
+    // Slot idx = (row, sample) with the sample fastest, matching the
+    // sample-fastest planes: lo = source offset inside a stage (pos * Mc + m,
+    // or sp * Mc + m for a point), -1 for padding; li = flat LDS index.
+    int sa[LA], sb[LB], su[LU], sv[LV], la[LA], lb[LB], lu[LU], lv[LV];
 #pragma unroll
     for (int k = 0; k < LA; k++) {
-        const int idx = tid + 256 * k, m = idx / ROWS, r = idx % ROWS, gu = ru0 + r / NR;
-        sa[k] = (idx < SUB * ROWS && gu < nurow) ? m * npos + urow_pos[gu] + r % NR : -1;
+        const int idx = tid + 256 * k, m = idx % SUB, r = idx / SUB, gu = ru0 + r / NR;
+        sa[k] = (idx < SUB * ROWS && gu < nurow) ? (urow_pos[gu] + r % NR) * Mc + m : -1;
+        la[k] = idx < SUB * ROWS ? m * ROWS + r : -1;
     }
 #pragma unroll
     for (int k = 0; k < LB; k++) {
-        const int idx = tid + 256 * k, m = idx / COLS, r = idx % COLS, gu = cu0 + r / NC;
-        sb[k] = (idx < SUB * COLS && gu < nucol) ? m * npos + ucol_pos[gu] + r % NC : -1;
+        const int idx = tid + 256 * k, m = idx % SUB, r = idx / SUB, gu = cu0 + r / NC;
+        sb[k] = (idx < SUB * COLS && gu < nucol) ? (ucol_pos[gu] + r % NC) * Mc + m : -1;
+        lb[k] = idx < SUB * COLS ? m * COLS + r : -1;
     }
 #pragma unroll
     for (int k = 0; k < LU; k++) {
-        const int idx = tid + 256 * k, rem = idx % (SUB * RU);
-        const int m = rem / RU, gu = ru0 + rem % RU;
-        su[k] = (idx < 3 * SUB * RU && gu < nurow) ? m * nsp_pad + urow_sp[gu] : -1;
+        const int idx = tid + 256 * k, m = idx % SUB, q = idx / SUB, f = q / RU, gu = ru0 + q % RU;
+        su[k] = (idx < 3 * SUB * RU && gu < nurow) ? urow_sp[gu] * Mc + m : -1;
+        lu[k] = idx < 3 * SUB * RU ? (f * SUB + m) * RU + q % RU : -1;
     }
 #pragma unroll
     for (int k = 0; k < LV; k++) {
-        const int idx = tid + 256 * k, rem = idx % (SUB * CU);
-        const int m = rem / CU, gu = cu0 + rem % CU;
-        sv[k] = (idx < 3 * SUB * CU && gu < nucol) ? m * nsp_pad + ucol_sp[gu] : -1;
+        const int idx = tid + 256 * k, m = idx % SUB, q = idx / SUB, f = q / CU, gu = cu0 + q % CU;
+        sv[k] = (idx < 3 * SUB * CU && gu < nucol) ? ucol_sp[gu] * Mc + m : -1;
+        lv[k] = idx < 3 * SUB * CU ? (f * SUB + m) * CU + q % CU : -1;
     }
     T va[LA], vb[LA], vc[LB], vd[LB], vu[LU], vv[LV];
     auto gload = [&](int m0) {
-        const T *c0 = comp + (int64_t)m0 * npos, *g0 = grp + (int64_t)m0 * nsp_pad;
+        const T *c0 = comp + m0, *g0 = grp + m0;
 #pragma unroll
         for (int k = 0; k < LA; k++) {
             va[k] = sa[k] >= 0 ? c0[sa[k]] : (T)0;
@@ -407,12 +449,12 @@ __global__ __launch_bounds__(256) void k_est_group(
         }
 #pragma unroll
         for (int k = 0; k < LU; k++) {
-            const int f = (tid + 256 * k) / (SUB * RU);
+            const int f = (tid + 256 * k) / SUB / RU;
             vu[k] = su[k] >= 0 ? g0[f * gplane + su[k]] : (f == 0 ? (T)1000.0 : (T)0);
         }
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const int f = (tid + 256 * k) / (SUB * CU);
+            const int f = (tid + 256 * k) / SUB / CU;
             vv[k] = sv[k] >= 0 ? g0[(3 + f) * gplane + sv[k]] : (f == 0 ? (T)-1000.0 : (T)0);
         }
     };
@@ -421,22 +463,22 @@ __global__ __launch_bounds__(256) void k_est_group(
         T *fU1 = &sU1[0][0][0], *fU2 = &sU2[0][0][0];
 #pragma unroll
         for (int k = 0; k < LA; k++)
-            if (tid + 256 * k < SUB * ROWS) {
-                fA1[tid + 256 * k] = va[k];
-                fB1[tid + 256 * k] = vb[k];
+            if (la[k] >= 0) {
+                fA1[la[k]] = va[k];
+                fB1[la[k]] = vb[k];
             }
 #pragma unroll
         for (int k = 0; k < LB; k++)
-            if (tid + 256 * k < SUB * COLS) {
-                fA2[tid + 256 * k] = vc[k];
-                fB2[tid + 256 * k] = vd[k];
+            if (lb[k] >= 0) {
+                fA2[lb[k]] = vc[k];
+                fB2[lb[k]] = vd[k];
             }
 #pragma unroll
         for (int k = 0; k < LU; k++)
-            if (tid + 256 * k < 3 * SUB * RU) fU1[tid + 256 * k] = vu[k];
+            if (lu[k] >= 0) fU1[lu[k]] = vu[k];
 #pragma unroll
         for (int k = 0; k < LV; k++)
-            if (tid + 256 * k < 3 * SUB * CU) fU2[tid + 256 * k] = vv[k];
+            if (lv[k] >= 0) fU2[lv[k]] = vv[k];
     };
     gload(0);
     for (int m0 = 0; m0 < Mround; m0 += SUB) {
@@ -932,15 +974,15 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                 ZV_HIPCHECK(c, hipGetLastError());
             } else {
                 const int *g = (const int *)c->b_gmeta.p;
-                const int *pos_pair = g, *pos_sp = g + c->grp_possp_off, *sp_first = pos_sp + np;
-                const unsigned nb = nblk((int64_t)Mc * np, 256);
+                const int *pos_pair = g, *sp_first = g + c->grp_possp_off + np;
+                const dim3 grid(nblk(Mc, 256) * (unsigned)c->nsp);
                 if (precision == ZV_PREC_FP32) {
                     float *comp = (float *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
-                        hipLaunchKernelGGL(k_dens_group<float>, dim3(nb), dim3(256), 0, c->stream,
-                                           B, PT, npp, pos_pair, pos_sp, sp_first, np,
-                                           c->grp_nsp_pad, Mv, Mc, d_r1, d_r2, d_pi0, comp, grp);
+                        hipLaunchKernelGGL(k_dens_group<float>, grid, dim3(256), 0, c->stream, B,
+                                           PT, npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1,
+                                           d_r2, d_pi0, comp, grp);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
@@ -949,9 +991,9 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                     double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
-                        hipLaunchKernelGGL(k_dens_group<double>, dim3(nb), dim3(256), 0, c->stream,
-                                           B, PT, npp, pos_pair, pos_sp, sp_first, np,
-                                           c->grp_nsp_pad, Mv, Mc, d_r1, d_r2, d_pi0, comp, grp);
+                        hipLaunchKernelGGL(k_dens_group<double>, grid, dim3(256), 0, c->stream, B,
+                                           PT, npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1,
+                                           d_r2, d_pi0, comp, grp);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
