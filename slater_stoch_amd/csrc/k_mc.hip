@@ -387,8 +387,12 @@ __global__ __launch_bounds__(256) void k_est_group(
     constexpr int SUB = GroupSub<T>::SUB;
     constexpr int RU = 16 * TR, CU = 16 * TC, ROWS = RU * NR, COLS = CU * NC;
     constexpr int ER = TR * NR, EC = TC * NC; // elements per thread
-    __shared__ T sA1[SUB][ROWS], sB1[SUB][ROWS], sA2[SUB][COLS], sB2[SUB][COLS];
-    __shared__ T sU1[3][SUB][RU], sU2[3][SUB][CU];
+    // Rows of one sample are padded so that the stage store, whose lanes run
+    // over SUB samples of one row, spreads over the LDS banks.
+    constexpr int PAD = sizeof(T) == 8 ? 8 : 4;
+    constexpr int ROWP = ROWS + PAD, COLP = COLS + PAD, RUP = RU + PAD, CUP = CU + PAD;
+    __shared__ T sA1[SUB][ROWP], sB1[SUB][ROWP], sA2[SUB][COLP], sB2[SUB][COLP];
+    __shared__ T sU1[3][SUB][RUP], sU2[3][SUB][CUP];
     const int2 tl = tiles[blockIdx.x];
     const int ru0 = tl.x * RU, cu0 = tl.y * CU;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -414,25 +418,25 @@ __global__ __launch_bounds__(256) void k_est_group(
     for (int k = 0; k < LA; k++) {
         const int idx = tid + 256 * k, m = idx % SUB, r = idx / SUB, gu = ru0 + r / NR;
         sa[k] = (idx < SUB * ROWS && gu < nurow) ? (urow_pos[gu] + r % NR) * Mc + m : -1;
-        la[k] = idx < SUB * ROWS ? m * ROWS + r : -1;
+        la[k] = idx < SUB * ROWS ? m * ROWP + r : -1;
     }
 #pragma unroll
     for (int k = 0; k < LB; k++) {
         const int idx = tid + 256 * k, m = idx % SUB, r = idx / SUB, gu = cu0 + r / NC;
         sb[k] = (idx < SUB * COLS && gu < nucol) ? (ucol_pos[gu] + r % NC) * Mc + m : -1;
-        lb[k] = idx < SUB * COLS ? m * COLS + r : -1;
+        lb[k] = idx < SUB * COLS ? m * COLP + r : -1;
     }
 #pragma unroll
     for (int k = 0; k < LU; k++) {
         const int idx = tid + 256 * k, m = idx % SUB, q = idx / SUB, f = q / RU, gu = ru0 + q % RU;
         su[k] = (idx < 3 * SUB * RU && gu < nurow) ? urow_sp[gu] * Mc + m : -1;
-        lu[k] = idx < 3 * SUB * RU ? (f * SUB + m) * RU + q % RU : -1;
+        lu[k] = idx < 3 * SUB * RU ? (f * SUB + m) * RUP + q % RU : -1;
     }
 #pragma unroll
     for (int k = 0; k < LV; k++) {
         const int idx = tid + 256 * k, m = idx % SUB, q = idx / SUB, f = q / CU, gu = cu0 + q % CU;
         sv[k] = (idx < 3 * SUB * CU && gu < nucol) ? ucol_sp[gu] * Mc + m : -1;
-        lv[k] = idx < 3 * SUB * CU ? (f * SUB + m) * CU + q % CU : -1;
+        lv[k] = idx < 3 * SUB * CU ? (f * SUB + m) * CUP + q % CU : -1;
     }
     T va[LA], vb[LA], vc[LB], vd[LB], vu[LU], vv[LV];
     auto gload = [&](int m0) {
