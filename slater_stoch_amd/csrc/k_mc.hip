@@ -376,13 +376,13 @@ template <> struct GroupSub<double> { static constexpr int SUB = 8; };
 // the reference's assignment (the electron-1 pair is the larger one), so
 // every quartet is written once.  Samples stream through LDS in stages of
 // SUB; stage sums in T are flushed to FP64.
-template <typename T, int NR, int NC, int TR, int TC>
+template <typename T, int NR, int NC, int TR, int TC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_est_group(
     const T *__restrict__ comp, const T *__restrict__ grp, const int *__restrict__ urow_pos,
     const int *__restrict__ urow_sp, int nurow, const int *__restrict__ ucol_pos,
     const int *__restrict__ ucol_sp, int nucol, const int2 *__restrict__ tiles,
-    const int *__restrict__ pos_pair, int Mround, int Mc, int npos, int nsp_pad,
-    double *__restrict__ acc)
+    const int *__restrict__ pos_pair, int Mround, int mspan, int Mc, int npos, int nsp_pad,
+    double *__restrict__ acc, double *__restrict__ mpart, int64_t ntri)
 {
     constexpr int SUB = GroupSub<T>::SUB;
     constexpr int RU = 16 * TR, CU = 16 * TC, ROWS = RU * NR, COLS = CU * NC;
@@ -484,11 +484,21 @@ __global__ __launch_bounds__(256) void k_est_group(
         for (int k = 0; k < LV; k++)
             if (lv[k] >= 0) fU2[lv[k]] = vv[k];
     };
-    gload(0);
-    for (int m0 = 0; m0 < Mround; m0 += SUB) {
+    // SPLIT: sample split blockIdx.y covers [ms0, ms1); split 0 adds into
+    // acc, split s > 0 into row s - 1 of mpart (folded in at the end of the
+    // block).  Without splits the chunk is walked whole (a separate
+    // instantiation: the general bounds cost the FP32 kernels 15 %).
+    int ms0 = 0, ms1 = Mround;
+    if constexpr (SPLIT) {
+        ms0 = (int)blockIdx.y * mspan;
+        ms1 = ms0 + mspan < Mround ? ms0 + mspan : Mround;
+        if (blockIdx.y > 0) acc = mpart + (size_t)(blockIdx.y - 1) * ntri;
+    }
+    if (ms0 < ms1) gload(ms0);
+    for (int m0 = ms0; m0 < ms1; m0 += SUB) {
         lstore();
         __syncthreads();
-        if (m0 + SUB < Mround) gload(m0 + SUB);
+        if (m0 + SUB < ms1) gload(m0 + SUB);
         // FP32 sums a stage in T and flushes it to FP64; FP64 accumulates
         // directly.
         T a[ER][EC];
@@ -592,12 +602,19 @@ __global__ __launch_bounds__(256) void k_est_group(
 // ------------------------------------------------------------------ statistics
 // End of a block (src/two_e_slater.F90:463-481): x = e_S/npts - e_G/npts,
 // sums of x and x^2, accumulators cleared.
+// With sample splits, their nextra partial rows are added to e_S (in split
+// order) and cleared.
 __global__ void k_block_end(int64_t ntri, double f, double *accS, double *accG, double *s1,
-                            double *s2, double *blk)
+                            double *s2, double *blk, int nextra, double *mpart)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntri) return;
-    double x = accG ? accS[t] * f - accG[t] * f : accS[t] * f;
+    double eS = accS[t];
+    for (int s = 0; s < nextra; s++) {
+        eS += mpart[(size_t)s * ntri + t];
+        mpart[(size_t)s * ntri + t] = 0.0;
+    }
+    double x = accG ? eS * f - accG[t] * f : eS * f;
     s1[t] = s1[t] + x;
     s2[t] = s2[t] + x * x;
     accS[t] = 0.0;
@@ -767,6 +784,24 @@ static int pick_chunk(zv_context *c, int64_t npts, int precision)
     return (int)mc;
 }
 
+// Sample splits of the group estimator: a molecule with few tiles would
+// leave most CUs idle while each tile walks the whole chunk, so the chunk's
+// stages are split over up to 64 workgroups per tile, aiming at about 1024
+// workgroups in all (about one resident round: 4 per CU by LDS), with at
+// least 4 stages per split and at most 1 GiB of partial rows.
+static int pick_split(zv_context *c, int Mc, int precision)
+{
+    int64_t tiles = 0;
+    for (int k = 0; k < 9; k++) tiles += c->grp_ntile[k];
+    if (tiles <= 0) return 1;
+    const int sub = precision == ZV_PREC_FP32 ? GroupSub<float>::SUB : GroupSub<double>::SUB;
+    int64_t n = (1024 + tiles - 1) / tiles;
+    n = std::min<int64_t>(n, 64);
+    n = std::min<int64_t>(n, std::max<int64_t>(1, Mc / sub / 4));
+    n = std::min<int64_t>(n, 1 + ((int64_t)1 << 27) / std::max<int64_t>(1, c->nint));
+    return (int)std::max<int64_t>(n, 1);
+}
+
 // ---- group estimator tables -------------------------------------------------
 // unit classes: 1, 3 or 6 components of one shell pair
 constexpr int kUnitRows[3] = {1, 3, 6};
@@ -848,20 +883,32 @@ template <typename T, int NR, int NC, int TR, int TC>
 static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const T *comp,
                            const T *grp, const int *urp, const int *urs, int nur, const int *ucp,
                            const int *ucs, int nuc, const int2 *tiles, const int *pos_pair,
-                           int Mround, int Mc, double *acc)
+                           int Mround, int nsplit, int mspan, int Mc, double *acc,
+                           double *mpart)
 {
-    hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC>), dim3(ntile), dim3(256), 0, st, comp,
-                       grp, urp, urs, nur, ucp, ucs, nuc, tiles, pos_pair, Mround, Mc, c->npair,
-                       c->grp_nsp_pad, acc);
+    if (nsplit > 1)
+        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, true>), dim3(ntile, (unsigned)nsplit),
+                           dim3(256), 0, st, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles,
+                           pos_pair, Mround, mspan, Mc, c->npair, c->grp_nsp_pad, acc, mpart,
+                           c->nint);
+    else
+        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, false>), dim3(ntile), dim3(256), 0, st,
+                           comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, pos_pair, Mround,
+                           mspan, Mc, c->npair, c->grp_nsp_pad, acc, mpart, c->nint);
 }
 
 // One launch per pair of unit classes; each is too small to fill the device
 // alone, so they run side by side on the main and the side streams (they
-// write disjoint quartets) and are joined back into the main stream.
+// write disjoint quartets) and are joined back into the main stream.  With
+// nsplit > 1 every tile is also split over the samples of the chunk.
 template <typename T>
 static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int Mround, int Mc,
-                               double *acc)
+                               int nsplit, double *acc, double *mpart)
 {
+    constexpr int SUB = GroupSub<T>::SUB;
+    const int nstage = Mround / SUB;
+    const int per = (nstage + nsplit - 1) / nsplit;
+    const int mspan = per * SUB;
     const int *g = (const int *)c->b_gmeta.p;
     const int *pos_pair = g;
     hipStream_t sts[1 + zv_context::NSIDE] = {c->stream};
@@ -886,7 +933,7 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
             const int2 *tiles = (const int2 *)g + c->grp_tile_off[3 * a + b];
 #define ZV_GROUP(NR, NC, TR, TC)                                                                  \
     launch_group_t<T, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
-                                      pos_pair, Mround, Mc, acc)
+                                      pos_pair, Mround, nsplit, mspan, Mc, acc, mpart)
             switch (3 * a + b) {
             case 0: ZV_GROUP(1, 1, 4, 4); break;
             case 1: ZV_GROUP(1, 3, 4, 2); break;
@@ -930,6 +977,9 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
     ZV_CHECK(zv_ensure(c, c->b_s1, sizeof(double) * ntri));
     ZV_CHECK(zv_ensure(c, c->b_s2, sizeof(double) * ntri));
     if (block_means) ZV_CHECK(zv_ensure(c, c->b_blk, sizeof(double) * ntri));
+    const int nsplit = precision == ZV_PREC_FP64_PARITY ? 1 : pick_split(c, Mc, precision);
+    if (nsplit > 1) ZV_CHECK(zv_ensure(c, c->b_mpart, sizeof(double) * (nsplit - 1) * ntri));
+    double *mpart = nsplit > 1 ? (double *)c->b_mpart.p : nullptr;
     double *accS = (double *)c->b_accS.p;
     double *accG = precision == ZV_PREC_FP64_PARITY ? (double *)c->b_accG.p : nullptr;
     double *s1 = (double *)c->b_s1.p, *s2 = (double *)c->b_s2.p;
@@ -939,6 +989,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
         if (accG) ZV_CHECK(zv_fill_dev(c, accG, ntri, 0.0));
         ZV_CHECK(zv_fill_dev(c, s1, ntri, 0.0));
         ZV_CHECK(zv_fill_dev(c, s2, ntri, 0.0));
+        if (mpart) ZV_CHECK(zv_fill_dev(c, mpart, (int64_t)(nsplit - 1) * ntri, 0.0));
     }
     Basis B = dev_basis(c);
     PairTab PT = dev_pairs(c);
@@ -990,7 +1041,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    ZV_CHECK(run_group_estimator<float>(c, comp, grp, Mround, Mc, accS));
+                    ZV_CHECK(run_group_estimator<float>(c, comp, grp, Mround, Mc, nsplit, accS, mpart));
                 } else {
                     double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
@@ -1001,7 +1052,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    ZV_CHECK(run_group_estimator<double>(c, comp, grp, Mround, Mc, accS));
+                    ZV_CHECK(run_group_estimator<double>(c, comp, grp, Mround, Mc, nsplit, accS, mpart));
                 }
             }
         }
@@ -1009,7 +1060,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
             PhaseTimer tm(c, PH_STATS);
             hipLaunchKernelGGL(k_block_end, dim3(nblk(ntri, 256)), dim3(256), 0, c->stream, ntri,
                                1.0 / (double)npts, accS, accG, s1, s2,
-                               block_means ? (double *)c->b_blk.p : nullptr);
+                               block_means ? (double *)c->b_blk.p : nullptr, nsplit - 1, mpart);
             ZV_HIPCHECK(c, hipGetLastError());
         }
         if (block_means)

@@ -75,6 +75,7 @@ struct zv_context {
     DevBuf b_s1, b_s2;                 // per quartet sums over blocks
     DevBuf b_out, b_out2;              // reference-order outputs
     DevBuf b_blk;                      // optional block means
+    DevBuf b_mpart;                    // estimator partial sums of sample splits 1..n-1
     DevBuf b_pp;                       // primitive-pair data
     DevBuf b_ppE;                      // primitive-pair Hermite coefficients
     DevBuf b_ppcnt;                    // kept primitive pairs per AO pair
