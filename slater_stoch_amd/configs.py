@@ -70,13 +70,17 @@ def large_ch(seed=40, ng0=20, a_zv=1.8) -> Basis:
                            ng0=ng0, a_zv=a_zv)
 
 
-def batch(seed=64, count=64, ng0=20, a_zv=1.8):
-    """``count`` molecules of 2-20 atoms with seeded, uneven sizes."""
+def batch(seed=64, count=64, ng0=20, a_zv=1.8, max_ao=150):
+    """``count`` molecules of 2-20 atoms with seeded, uneven sizes, built as
+    ``random_molecule`` (heavy atoms 15 AOs, H 5 AOs).  The number of heavy
+    atoms is capped so that no molecule exceeds ``max_ao`` orbitals
+    (10-150 AOs for the defaults)."""
     rng = np.random.default_rng(seed)
     out = []
     for t in range(count):
         natom = int(rng.integers(2, 21))
-        nh = int(rng.integers(0, natom + 1))
+        max_heavy = max(0, min(natom, (max_ao - 5 * natom) // 10))
+        nh = natom - int(rng.integers(0, max_heavy + 1))
         out.append(random_molecule(seed=int(rng.integers(1 << 30)), n_heavy=natom - nh, n_h=nh,
                                    box=3.0 + 0.35 * natom, ng0=ng0, a_zv=a_zv))
     return out
