@@ -12,7 +12,16 @@
 // converged triples are deflated from W_work and negative ones removed from
 // W; passes stop when N vectors are accepted or max|D| < 1e-12.  Finally
 // moy = (1/8) sum of the eight permutations (src/svd_clean.F90:372-405).
-// GEMMs are rocBLAS DGEMM, QR rocSOLVER geqrf/orgqr, SVD rocSOLVER gesvd.
+// GEMMs are rocBLAS DGEMM.  Orthonormal bases come from shifted CholeskyQR3
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
+// 42 (2020) A477: one Cholesky QR of the shifted Gram matrix, then two plain
+// ones; DGEMM + a 512 x 512 Cholesky + TRSM per round) with rocSOLVER
+// geqrf/orgqr as the fallback when a Cholesky factorization fails (rank-
+// deficient panels).  The SVD of the wide r x N matrix Y = Z^T W_work goes
+// through an orthonormal basis Q of Y^T: Y = (Y Q) Q^T, and the r x r factor
+// Y Q is decomposed by rocSOLVER gesvdj (one-sided Jacobi).  Both give the same subspace and the
+// same singular triples as the reference's Householder QR and full SVD; they
+// only avoid the tall-skinny factorizations that dominate on the GPU.
 //
 // zv_davidson_clean is the dense branch of src/davidson_clean.F90:30-120:
 // symmetric eigendecomposition (rocSOLVER syevd) and W -= sum_{e<0} e u u^T.
@@ -183,6 +192,48 @@ __global__ void k_colscale(int64_t N, int r, const double *U, const double *D, d
     out[t] = U[t] * f;
 }
 
+// G += (f * trace G) I, one workgroup: the shift of the first CholeskyQR
+// round (trace G = ||Z||_F^2 bounds ||Z||_2^2 from above)
+__global__ void k_shift_diag(double *G, int r, double f)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < r; k += 256) s += G[(int64_t)k * r + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double shift = f * red[0];
+    for (int k = threadIdx.x; k < r; k += 256) G[(int64_t)k * r + k] += shift;
+}
+
+// flag = 1 when the Cholesky factor R (upper, in G) has min R_kk < thr max
+// R_kk (or a non-positive / non-finite pivot), else 0
+__global__ void k_pivot_check(const double *G, int r, double thr, rocblas_int *flag)
+{
+    __shared__ double mn[256], mx[256];
+    double lo = INFINITY, hi = 0.0;
+    for (int k = threadIdx.x; k < r; k += 256) {
+        const double d = G[(int64_t)k * r + k];
+        lo = d < lo ? d : lo;
+        hi = d > hi ? d : hi;
+        if (!(d > 0.0) || !isfinite(d)) lo = -1.0;
+    }
+    mn[threadIdx.x] = lo;
+    mx[threadIdx.x] = hi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + o]);
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *flag = (mn[0] > thr * mx[0]) ? 0 : 1;
+}
+
 } // namespace
 
 // one rocBLAS handle per context, bound to the context's stream
@@ -232,6 +283,57 @@ static int qr_inplace(zv_context *c, rocblas_handle h, double *Z, int64_t N, int
     return ZV_OK;
 }
 
+// Scratch of the orthonormalization: G (r x r), a copy of the panel
+// (N x r), Householder tau (r), three Cholesky info words and two pivot
+// flags.
+struct OrthoWork {
+    double *G, *backup, *tau;
+    rocblas_int *info;
+};
+
+// Z (N x r, N >= r) <- an orthonormal basis of its columns, in place:
+// shifted CholeskyQR3; Householder QR from the saved panel when one of the
+// three Cholesky factorizations fails, or when a factor of the unshifted
+// rounds has a pivot ratio below 1e-6.  For a panel with condition number
+// below 1/u the shifted round leaves cond(Q1) <= ~1/sqrt(11 N r u) (pivot
+// ratio >= 6.7e-6 at N r = 3.5e4, >= 1.6e-4 at 2e7); a rank-deficient panel
+// gives ratios near sqrt(u) = 1e-8, where a "successful" Cholesky would
+// return columns outside the panel's range.
+static int orthonormalize(zv_context *c, rocblas_handle h, double *Z, int64_t N, int r,
+                          const OrthoWork &w, int *n_fallback)
+{
+    const double one = 1.0, zero = 0.0;
+    const double f = 11.0 * ((double)N * r + (double)r * (r + 1)) * 0x1.0p-53;
+    ZV_HIPCHECK(c, hipMemcpyAsync(w.backup, Z, sizeof(double) * N * r, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    for (int round = 0; round < 3; round++) {
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r,
+                            (rocblas_int)N, &one, Z, (rocblas_int)N, Z, (rocblas_int)N, &zero,
+                            w.G, r));
+        if (round == 0) {
+            hipLaunchKernelGGL(k_shift_diag, dim3(1), dim3(256), 0, c->stream, w.G, r, f);
+            ZV_HIPCHECK(c, hipGetLastError());
+        }
+        RB(c, rocsolver_dpotrf(h, rocblas_fill_upper, r, w.G, r, w.info + round));
+        if (round > 0) {
+            hipLaunchKernelGGL(k_pivot_check, dim3(1), dim3(256), 0, c->stream, (const double *)w.G,
+                               r, 1.0e-6, w.info + 2 + round);
+            ZV_HIPCHECK(c, hipGetLastError());
+        }
+        RB(c, rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                            rocblas_diagonal_non_unit, (rocblas_int)N, r, &one, w.G, r, Z,
+                            (rocblas_int)N));
+    }
+    rocblas_int hinfo[5] = {0, 0, 0, 0, 0};
+    ZV_HIPCHECK(c, hipMemcpyAsync(hinfo, w.info, sizeof(hinfo), hipMemcpyDeviceToHost, c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (!(hinfo[0] | hinfo[1] | hinfo[2] | hinfo[3] | hinfo[4])) return ZV_OK;
+    ZV_HIPCHECK(c, hipMemcpyAsync(Z, w.backup, sizeof(double) * N * r, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    if (n_fallback) (*n_fallback)++;
+    return qr_inplace(c, h, Z, N, r, w.tau);
+}
+
 int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri, int *n_removed)
 {
     const int64_t N = (int64_t)c->n * c->n;
@@ -240,14 +342,21 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
     rocblas_handle h;
     ZV_CHECK(blas_handle(c, &h));
     PhaseTimer tm(c, PH_CLEAN);
-    // workspace: W, Wwork (N^2), P, Z, U, S (N r), Y/Vt (r N), UY (r^2), small vectors
-    size_t nn = (size_t)N * N, nr = (size_t)N * r;
-    size_t words = 2 * nn + 4 * nr + 2 * nr + (size_t)r * r + 8 * (size_t)r + c->nint + 16;
+    // workspace: W, Wwork (N^2), P, Z, U, S (N r), Yt/Vt (N r), UY, Rt, Vs, G (r^2),
+    // small vectors
+    size_t nn = (size_t)N * N, nr = (size_t)N * r, rr = (size_t)r * r;
+    size_t words = 2 * nn + 6 * nr + 4 * rr + 8 * (size_t)r + c->nint + 16;
     ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
     double *W = (double *)c->b_misc.p, *Ww = W + nn, *P = Ww + nn, *Z = P + nr, *U = Z + nr,
-           *S = U + nr, *Y = S + nr, *Vt = Y + nr, *UY = Vt + nr, *D = UY + (size_t)r * r,
-           *E = D + r, *tau = E + r, *dots = tau + r, *d_eri = dots + r + 4;
+           *S = U + nr, *Yt = S + nr, *Vt = Yt + nr, *UY = Vt + nr, *Rt = UY + rr, *Vs = Rt + rr,
+           *G = Vs + rr, *D = G + rr, *E = D + r, *tau = E + r, *dots = tau + r,
+           *d_eri = dots + r + 4;
+    // info words: [0] SVD, [1, 6) orthonormalization, [6] Jacobi sweeps
     rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    // power iterations keep their panel copy in S, the SVD's in U (both are
+    // written only after the SVD)
+    const OrthoWork ow_iter{G, S, tau, info + 1}, ow_svd{G, U, tau, info + 1};
+    int n_fallback = 0;
     ZV_CHECK(build_W(c, eri, d_eri, Ww));
     ZV_HIPCHECK(c, hipMemcpyAsync(W, Ww, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
     const double one = 1.0, zero = 0.0, mone = -1.0;
@@ -272,19 +381,31 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
         ZV_HIPCHECK(c, hipGetLastError());
         RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, N, &one, Ww, N,
                             P, N, &zero, Z, N));
-        ZV_CHECK(qr_inplace(c, h, Z, N, r, tau));
+        ZV_CHECK(orthonormalize(c, h, Z, N, r, ow_iter, &n_fallback));
         for (int it = 0; it < q; it++) {
             RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, N, r, N,
                                 &one, Ww, N, Z, N, &zero, P, N));
             RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, N, &one,
                                 Ww, N, P, N, &zero, Z, N));
-            ZV_CHECK(qr_inplace(c, h, Z, N, r, tau));
+            ZV_CHECK(orthonormalize(c, h, Z, N, r, ow_iter, &n_fallback));
         }
-        // Y (r x N) = Z^T W_work
-        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, N, N, &one,
-                            Z, N, Ww, N, &zero, Y, r));
-        RB(c, rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, r, (rocblas_int)N,
-                               Y, r, D, UY, r, Vt, r, E, rocblas_outofplace, info));
+        // SVD of Y = Z^T W_work (r x N): Yt = Y^T, Q = orthonormal basis of
+        // Yt (in S), Rt = Y Q (r x r) = UY diag(D) Vs, so Y = UY diag(D) (Vs Q^T)
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, N, r, N, &one,
+                            Ww, N, Z, N, &zero, Yt, N));
+        ZV_HIPCHECK(c, hipMemcpyAsync(S, Yt, sizeof(double) * nr, hipMemcpyDeviceToDevice,
+                                      c->stream));
+        ZV_CHECK(orthonormalize(c, h, S, N, r, ow_svd, &n_fallback));
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r, N, &one,
+                            Yt, N, S, N, &zero, Rt, r));
+        // one-sided Jacobi: accurate singular vectors inside the clusters of
+        // the late passes (rocSOLVER's divide-and-conquer gesdd lost the
+        // small negative eigenvalues of a 110-AO ERI matrix there; gesvd is
+        // accurate but 660 ms at r = 512)
+        RB(c, rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, r, r, Rt, r,
+                                0.0, E, 100, info + 6, D, UY, r, Vs, r, info));
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, N, r, &one,
+                            Vs, r, S, N, &zero, Vt, r));
         RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, r, r, &one, Z, N,
                             UY, r, &zero, U, N));
         hipLaunchKernelGGL(k_coldot, dim3(r), dim3(256), 0, c->stream, N, r, U, Vt, dots);
@@ -301,11 +422,13 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
         if (c->verbose) {
             printf(" Largest/Smallest singular value : %g %g\n", (double)(float)hD[0],
                    (double)(float)hD[r - 1]);
+            if (n_fallback) printf(" Householder QR fallbacks so far: %d\n", n_fallback);
             fflush(stdout);
         }
         // src/svd_clean.F90:286-296
         npass += r;
         double dmax = 0.0;
+        int negative = 0;
         for (int k = 0; k < r; k++) {
             if (hdots[k] < 0.0) hD[k] = -hD[k];
             if (fabs(hdots[k]) < 0.99) {
@@ -313,8 +436,9 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
                 npass -= 1;
             }
             dmax = std::max(dmax, fabs(hD[k]));
-            if (hD[k] < 0.0) removed++;
+            if (hD[k] < 0.0) negative++;
         }
+        removed += negative;
         ZV_HIPCHECK(c, hipMemcpyAsync(D, hD.data(), sizeof(double) * r, hipMemcpyHostToDevice,
                                       c->stream));
         // W_work -= U |D| Vt ; W += U min(D,0) Vt  (src/svd_clean.F90:334-353)
@@ -322,10 +446,12 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
                            S, 0);
         RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, N, r, &mone, S,
                             N, Vt, r, &one, Ww, N));
-        hipLaunchKernelGGL(k_colscale, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, N, r, U, D,
-                           S, 1);
-        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, N, r, &one, S,
-                            N, Vt, r, &one, W, N));
+        if (negative) { // min(D, 0) = 0 for every triple otherwise
+            hipLaunchKernelGGL(k_colscale, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, N, r, U,
+                               D, S, 1);
+            RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, N, r, &one,
+                                S, N, Vt, r, &one, W, N));
+        }
         ZV_HIPCHECK(c, hipGetLastError());
         if (dmax < 1.0e-12) break; // src/svd_clean.F90:359
     }
