@@ -122,9 +122,10 @@ int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
 /* Progress lines of the long-running calls, written to stdout: level 1
  * prints what the reference prints unconditionally in svd_clean ("Vectors
  * a -- b / N" per randomized pass, src/svd_clean.F90:153-154, and the
- * largest/smallest singular value of the pass, :285); level 0 (the default)
- * prints nothing.  No counterpart in the reference, whose prints cannot be
- * switched off. */
+ * largest/smallest singular value of the pass, :285); level 2 adds one line
+ * per finished zv_mc_blocks block (the call then waits for each block, so
+ * the line reports work done); level 0 (the default) prints nothing.  No
+ * counterpart in the reference, whose prints cannot be switched off. */
 int zv_set_verbose(zv_context *ctx, int level);
 
 /* Milliseconds spent in the last call's kernels, by phase:

@@ -1066,6 +1066,12 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
         if (block_means)
             ZV_CHECK(zv_run_scatter_ref(c, (const double *)c->b_blk.p,
                                         block_means + (size_t)b * ntri));
+        if (c->verbose >= 2) {
+            ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+            printf(" MC block %d / %d done (%lld samples, %lld quartets)\n", b + 1, nbl,
+                   (long long)npts, (long long)ntri);
+            fflush(stdout);
+        }
     }
     ZV_CHECK(zv_ensure(c, c->b_out2, sizeof(double) * ntri));
     {

@@ -1,0 +1,95 @@
+"""The batch driver: sample-size policy and worker independence.
+
+CPU tests drive run_molecule with a stand-in engine whose standard error
+falls as 1/sqrt(samples); the GPU test runs real molecules and checks that
+the batch gives the same bits as one-molecule runs with the same seeds,
+whatever the number of worker threads."""
+import numpy as np
+import pytest
+
+from slater_stoch_amd import batch as B
+from slater_stoch_amd import configs
+
+
+def test_next_npts_rounds_scales_and_caps():
+    # already below target: no further run
+    assert B.next_npts(4000, 5e-7, 1e-6, 1.2, 10**6) == 0
+    # (2e-6 / 1e-6)^2 x 1.2 x 4000 = 19200, a multiple of 40
+    assert B.next_npts(4000, 2e-6, 1e-6, 1.2, 10**6) == 19200
+    # rounded up to a multiple of 40
+    assert B.next_npts(4000, 1.01e-6, 1e-6, 1.0, 10**6) % 40 == 0
+    # at least one walker block more than the last run
+    assert B.next_npts(4000, 1.0000001e-6, 1e-6, 1.0, 10**6) >= 4040
+    # capped by the budget, rounded down to a multiple of 40
+    assert B.next_npts(4000, 1e-4, 1e-6, 1.2, 123456) == 123440
+    # no run when the budget cannot exceed the last run
+    assert B.next_npts(4000, 1e-4, 1e-6, 1.2, 4000) == 0
+
+
+class FakeEngine:
+    """err = s0 / sqrt(nbl * npts) for every quartet; records the calls."""
+
+    def __init__(self, s0, nint=5):
+        self.s0, self.nint, self.calls = s0, nint, []
+
+    def set_basis(self, bas):
+        self.bas = bas
+
+    def gauss_eri(self):
+        return np.zeros(self.nint), None, 0
+
+    def mc_blocks(self, nbl, npts, precision, seed=0, stream=0):
+        self.calls.append((nbl, npts, precision, seed, stream))
+        err = np.full(self.nint, self.s0 / np.sqrt(nbl * npts))
+        return np.full(self.nint, float(stream)), err
+
+
+class FakeBasis:
+    n, nint = 3, 5
+
+
+def test_policy_reaches_target_in_two_runs():
+    # pilot 40000 samples -> 2e-6; the second run is sized for 1e-6 x sqrt(1/1.2)
+    eng = FakeEngine(s0=2e-6 * np.sqrt(40000))
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6, precision="fp32", seed=7)
+    assert r["rounds"] == 2 and r["reached"]
+    assert [c[4] for c in eng.calls] == [0, 1]          # one Philox stream per run
+    assert all(c[3] == 7 for c in eng.calls)
+    assert eng.calls[1][1] == 19200
+    assert r["samples"] == 10 * 4000 + 10 * 19200
+    assert r["max_stderr"] == pytest.approx(2e-6 * np.sqrt(40000 / 192000))
+
+
+def test_policy_stops_at_the_sample_budget():
+    eng = FakeEngine(s0=1e-3 * np.sqrt(40000))          # hopeless: 1e-3 after the pilot
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6, max_samples=1_000_000)
+    assert not r["reached"]
+    assert r["samples"] <= 1_000_000
+    assert r["rounds"] == 2                              # pilot + what the budget allows
+    assert eng.calls[1][1] == (1_000_000 - 40000) // 10 // 40 * 40
+
+
+def test_policy_skips_production_when_the_pilot_suffices():
+    eng = FakeEngine(s0=1e-7 * np.sqrt(40000))
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6)
+    assert r["rounds"] == 1 and r["reached"] and r["samples"] == 40000
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_runs_for_any_worker_count():
+    from slater_stoch_amd.zvmc import Engine
+
+    bases = [b for b in configs.batch() if b.n <= 30][:4]
+    assert len(bases) >= 3
+    pol = dict(nbl=4, pilot_npts=400, max_samples=200_000, max_rounds=3, keep_arrays=True)
+    one = B.run_batch(bases, target=1e-7, precision="fp64", workers=1, seed=11, **pol)
+    three = B.run_batch(bases, target=1e-7, precision="fp64", workers=3, seed=11, **pol)
+    with Engine(0) as e:
+        for t, b in enumerate(bases):
+            ref = B.run_molecule(e, b, target=1e-7, precision="fp64", seed=11 + t, **pol)
+            for got in (one[t], three[t]):
+                assert got["rounds"] == ref["rounds"] and got["samples"] == ref["samples"]
+                np.testing.assert_array_equal(got["eri"], ref["eri"])
+                np.testing.assert_array_equal(got["stderr"], ref["stderr"])
+    # the tight target forces production runs on at least one molecule
+    assert max(r["rounds"] for r in one) >= 2
