@@ -362,10 +362,12 @@ __global__ void k_dens_group(Basis B, PairTab Tb, int npp, const int *__restrict
     }
 }
 
-// samples per LDS stage of the group estimator (deeper stages cost more LDS
-// than they gain in latency hiding: measured on MI355X)
+// samples per LDS stage of the group estimator, measured on MI355X: deeper
+// stages cost more LDS than they gain in latency hiding, and a 16-sample FP32
+// stage holds more prefetch registers than two waves per SIMD allow (FP32
+// estimator 13.5 ms per block at 16, 11.8 ms at 8 on the 110-AO molecule)
 template <typename T> struct GroupSub;
-template <> struct GroupSub<float> { static constexpr int SUB = 16; };
+template <> struct GroupSub<float> { static constexpr int SUB = 8; };
 template <> struct GroupSub<double> { static constexpr int SUB = 8; };
 
 // A workgroup owns a tile of 16 TR row units x 16 TC column units (units of
