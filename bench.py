@@ -135,12 +135,9 @@ def roofline_eri(res):
             "all_eri_kernels_tflops": total_flops / (total_ms / 1e3) / 1e12}
 
 
-def shell_units(bas):
-    """The library's shells (consecutive AOs with equal centre, fit, type and
-    order, at most 15) and its split of every shell pair's components into
-    units of 6, 3 and 1 (csrc/k_mc.hip, group_ready): returns, per shell pair
-    (I, K), the first shell I and the list of units as arrays of pair
-    indices."""
+def shell_firsts(bas):
+    """First AO of every shell of the library (consecutive AOs with equal
+    centre, fit, type and order, at most 15), then n."""
     n = bas.n
     lao = bas.npower.sum(1) + 2 * (bas.i_type == 2)
     first = []
@@ -153,6 +150,14 @@ def shell_units(bas):
         if not same:
             first.append(i)
     first.append(n)
+    return first
+
+
+def shell_units(bas):
+    """The library's split of every shell pair's components into units of 6,
+    3 and 1 (csrc/k_mc.hip, group_ready): returns, per shell pair (I, K), the
+    first shell I and the list of units as arrays of pair indices."""
+    first = shell_firsts(bas)
     out = []
     for I in range(len(first) - 1):
         for K in range(I + 1):
@@ -216,6 +221,47 @@ def roofline_estimator(res, bas, prec):
             "avg_launch_ms": ms / max(1, n), "flops_per_launch": flops / max(1, n),
             "quartet_samples_per_s": float(bas.nint) * samples / (ms / 1e3),
             "kernel_evaluations_per_s": nup * samples / (ms / 1e3)}
+
+
+def density_report(res, bas, prec):
+    """The density kernel (one thread per shell pair and sample): per shell
+    pair and sample it evaluates the STO and its fit for the first AO of both
+    shells at both points -- 4 (1 + ng) exponentials -- and writes four
+    component values per pair plus six point coordinates per shell pair."""
+    ms, n = res["mc_timings"]["densities"]
+    samples = NBL * res["production_npts"]
+    first = shell_firsts(bas)
+    ng = [int(bas.n_gauss[first[I]]) for I in range(len(first) - 1)]
+    nsh = len(ng)
+    exps = sum(2 * ((1 + ng[I]) + (1 + ng[K])) for I in range(nsh) for K in range(I + 1))
+    nsp = nsh * (nsh + 1) // 2
+    width = 4 if prec == "fp32" else 8
+    written = width * (4 * bas.npair + 6 * nsp) * samples
+    sec = ms / 1e3
+    return {"kernel": f"k_dens_group<{'float' if prec == 'fp32' else 'double'}>",
+            "launches": n, "total_ms": ms, "exp_per_s": exps * samples / sec,
+            "bytes_written": written, "write_GBps": written / sec / 1e9,
+            "write_frac_of_hbm_peak": written / sec / 8e12}
+
+
+def eri_by_class(res):
+    """Per (LU, LL) shell class of the analytic phase (both lane batches and
+    modes together): kernel ms, primitive quartet x lane component
+    evaluations per second, algorithmic FP64 TFLOP/s and its fraction of the
+    78.6 TFLOP/s peak; heaviest first."""
+    agg = {}
+    for key, d in res["eri_stats"].items():
+        k = (key[1], key[2]) if key[0] == "shell" else ("component", key[1], key[2])
+        a = agg.setdefault(k, {"ms": 0.0, "units": 0.0, "flops": 0.0})
+        for f in ("ms", "units", "flops"):
+            a[f] += d[f]
+    out = []
+    for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["ms"]):
+        sec = max(a["ms"], 1e-9) / 1e3
+        out.append({"class": list(k), "ms": a["ms"], "prim_quartet_components_per_s":
+                    a["units"] / sec, "tflops": a["flops"] / sec / 1e12,
+                    "frac": a["flops"] / sec / 1e12 / PEAK_FP64})
+    return out
 
 
 def roofline_of(res, bas):
@@ -360,6 +406,8 @@ def main():
         "mfma_fraction": 0.0,
         "roofline": roof,
         "roofline_other_phase": roof_other,
+        "density_phase": density_report(last, bas, "fp64"),
+        "eri_by_class": eri_by_class(last)[:12],
         "cpu_baseline": cpu,
     }
     if cpu:
