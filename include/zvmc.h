@@ -78,6 +78,17 @@ int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls);
  * Per quartet this is gauss_ijkl (src/gauss_ijkl.F90:51-154). */
 int zv_gauss_eri(zv_context *ctx, double *ijkl_gaus, double *precond, int64_t *n_screened);
 
+/* Exact Slater integrals of the one-centre quartets (all four AOs on one
+ * atom), the do_exact_monocenter branch of src/two_e_slater.F90:318-365
+ * (per quartet src/compute_int_slater.F90 -> src/ijkl_slater_xnynzn.F90):
+ * ijkl[kcp] (in/out, nint) is overwritten with the exact value for every
+ * one-centre kcp and err[kcp] (in/out, may be NULL) with 0, as the
+ * reference's final assembly does (src/two_e_slater.F90:489-493, 558-563);
+ * other entries are left as they are.  n_mono (out, may be NULL) = number
+ * of one-centre quartets.  AOs above d (4F, 5G) on any atom give
+ * ZV_ERR_ARG, where the reference stops. */
+int zv_exact_monocenter(zv_context *ctx, double *ijkl, double *err, int64_t *n_mono);
+
 /* gauss_ijkl(i,k,j,l) for an explicit list of nq quartets (1-based orbital
  * indices; no screening).  src/gauss_ijkl.F90:51 batched. */
 int zv_gauss_ijkl_list(zv_context *ctx, int64_t nq, const int *qi, const int *qk,

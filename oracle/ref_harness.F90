@@ -131,6 +131,16 @@ contains
     ref_gauss_ijkl = gauss_ijkl(i, k, j, l)
   end function
 
+  ! Exact one-centre Slater integral (ik|jl) of the loaded basis, 1-based
+  ! indices (src/compute_int_slater.F90, the do_exact_monocenter branch of
+  ! src/two_e_slater.F90:350-365).
+  real(c_double) function ref_int_slater(i, k, j, l) bind(C, name="ref_int_slater")
+    integer(c_int), value :: i, k, j, l
+    real(c_double) :: v
+    call compute_int_slater(int(i), int(k), int(j), int(l), v)
+    ref_int_slater = v
+  end function
+
   real(c_double) function ref_rint(n, rho) bind(C, name="ref_rint")
     integer(c_int), value :: n
     real(c_double), value :: rho

@@ -873,3 +873,185 @@ void zvo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
+
+/* ------------------------------------------------------------------ */
+/* Exact one-centre Slater integrals (the do_exact_monocenter branch,   */
+/* src/two_e_slater.F90:318-365; src/compute_int_slater.F90,            */
+/* src/ijkl_slater_xnynzn.F90, src/ijkl_slater.F90, src/bigF.F90,       */
+/* src/smallg.F90, src/compute_lmin_lmax.F90, src/gaunt.F90).           */
+/*                                                                    */
+/* (ik|jl) for four STOs x^a y^b z^c r^nsto e^{-zeta r} on one centre:  */
+/* each angular factor (x/r)^a (y/r)^b (z/r)^c (degree <= 2) is expanded */
+/* in complex spherical harmonics (Condon-Shortley phase), the radial   */
+/* part is r^(n-1) with n = 1 + a + b + c + nsto, and                    */
+/* (ik|jl) = Re sum c_i* c_k c_j* c_l 4 pi sum_L G G F_L / (2L+1)        */
+/* with Gaunt coefficients G and the closed-form radial Coulomb          */
+/* integral F_L.  The expansion coefficients are computed here by exact  */
+/* quadrature of int Y*_lm (x/r)^a (y/r)^b (z/r)^c dOmega.               */
+/* ------------------------------------------------------------------ */
+#include <complex.h>
+
+#define ZVO_PI 3.14159265358979323846
+
+static double ofact(int n)
+{
+    double f = 1.0;
+    for (int i = 2; i <= n; i++) f *= (double)i;
+    return f;
+}
+
+/* Wigner 3j symbol by Racah's formula */
+static double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3)
+{
+    if (m1 + m2 + m3 != 0) return 0.0;
+    if (j3 < abs(j1 - j2) || j3 > j1 + j2) return 0.0;
+    if (abs(m1) > j1 || abs(m2) > j2 || abs(m3) > j3) return 0.0;
+    const double tri = ofact(j1 + j2 - j3) * ofact(j1 - j2 + j3) * ofact(-j1 + j2 + j3) /
+                       ofact(j1 + j2 + j3 + 1);
+    const double pre = sqrt(tri * ofact(j1 + m1) * ofact(j1 - m1) * ofact(j2 + m2) *
+                            ofact(j2 - m2) * ofact(j3 + m3) * ofact(j3 - m3));
+    int kmin = 0, kmax = j1 + j2 - j3;
+    if (j2 - j3 - m1 > kmin) kmin = j2 - j3 - m1;
+    if (j1 - j3 + m2 > kmin) kmin = j1 - j3 + m2;
+    if (j1 - m1 < kmax) kmax = j1 - m1;
+    if (j2 + m2 < kmax) kmax = j2 + m2;
+    double s = 0.0;
+    for (int k = kmin; k <= kmax; k++)
+        s += ((k & 1) ? -1.0 : 1.0) /
+             (ofact(k) * ofact(j3 - j2 + k + m1) * ofact(j3 - j1 + k - m2) *
+              ofact(j1 + j2 - j3 - k) * ofact(j1 - k - m1) * ofact(j2 - k + m2));
+    return ((abs(j1 - j2 - m3) & 1) ? -1.0 : 1.0) * pre * s;
+}
+
+/* int dOmega Y*_{l1 m1} Y_{l2 m2} Y_{l3 m3} */
+double zvo_gaunt(int l1, int l2, int l3, int m1, int m2, int m3)
+{
+    const double f = sqrt((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0) * (2.0 * l3 + 1.0) / (4.0 * ZVO_PI));
+    return ((abs(m1) & 1) ? -1.0 : 1.0) * f * wigner3j(l1, l2, l3, 0, 0, 0) *
+           wigner3j(l1, l2, l3, -m1, m2, m3);
+}
+
+/* associated Legendre function without the Condon-Shortley phase, l <= 2 */
+static double plm(int l, int m, double x)
+{
+    const double s = sqrt(fmax(0.0, 1.0 - x * x));
+    if (l == 0) return 1.0;
+    if (l == 1) return m == 0 ? x : s;
+    return m == 0 ? 0.5 * (3.0 * x * x - 1.0) : (m == 1 ? 3.0 * x * s : 3.0 * (1.0 - x * x));
+}
+
+/* Y_lm(theta, phi) with the Condon-Shortley phase (-1)^m for m > 0, l <= 2 */
+double complex zvo_ylm(int l, int m, double ct, double phi)
+{
+    const int am = abs(m);
+    const double c = sqrt((2.0 * l + 1.0) / (4.0 * ZVO_PI) * ofact(l - am) / ofact(l + am));
+    const double sign = (m > 0 && (m & 1)) ? -1.0 : 1.0;
+    return sign * c * plm(l, am, ct) * cexp(I * (double)m * phi);
+}
+
+/* expansion coefficients c[a][b][c][l][m+2] of (x/r)^a (y/r)^b (z/r)^c,
+ * a + b + c <= 2, by Gauss-Legendre (cos theta, 8 nodes) x trapezoid (phi,
+ * 16 nodes): exact for these polynomial integrands */
+static double complex g_ylm_c[3][3][3][3][5];
+static int g_ylm_ready = 0;
+
+static void ylm_expansion(void)
+{
+    static const double xg[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                                 -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
+                                 0.7966664774136267,  0.9602898564975363};
+    static const double wg[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
+                                 0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
+                                 0.2223810344533745, 0.1012285362903763};
+    memset(g_ylm_c, 0, sizeof(g_ylm_c));
+    for (int a = 0; a <= 2; a++)
+        for (int b = 0; a + b <= 2; b++)
+            for (int c = 0; a + b + c <= 2; c++)
+                for (int l = 0; l <= a + b + c; l++)
+                    for (int m = -l; m <= l; m++) {
+                        double complex s = 0.0;
+                        for (int t = 0; t < 8; t++) {
+                            const double ct = xg[t], st = sqrt(1.0 - ct * ct);
+                            for (int p = 0; p < 16; p++) {
+                                const double phi = 2.0 * ZVO_PI * p / 16.0;
+                                const double x = st * cos(phi), y = st * sin(phi), z = ct;
+                                const double f = pow(x, a) * pow(y, b) * pow(z, c);
+                                s += wg[t] * (2.0 * ZVO_PI / 16.0) * f * conj(zvo_ylm(l, m, ct, phi));
+                            }
+                        }
+                        g_ylm_c[a][b][c][l][m + 2] = s;
+                    }
+    g_ylm_ready = 1;
+}
+
+void zvo_ylm_coef(int a, int b, int c, int l, int m, double *re, double *im)
+{
+    if (!g_ylm_ready) ylm_expansion();
+    const double complex v = g_ylm_c[a][b][c][l][m + 2];
+    *re = creal(v);
+    *im = cimag(v);
+}
+
+/* F_L = int r1^(k1) e^{-g1 r1} [ r1^-(L+1) int_0^r1 r2^(k2+L) e^{-g2 r2} dr2
+ *                              + r1^L int_r1^inf r2^(k2-L-1) e^{-g2 r2} dr2 ] dr1,
+ * k1 = n1 + n2, k2 = n3 + n4 (>= L + 1), in closed form */
+static double radial_F(int k1, double g1, int k2, double g2, int L)
+{
+    const double g = g1 + g2;
+    double v = ofact(k2 + L) / pow(g2, k2 + L + 1) * ofact(k1 - L - 1) / pow(g1, k1 - L);
+    /* S(p) = sum_{i=0}^{k2+p} (k2+p)! (k1+k2-1-i)! / (k2+p-i)! / g2^(i+1) / g^(k1+k2-i) */
+    for (int sgn = 0; sgn < 2; sgn++) {
+        const int p = sgn == 0 ? -L - 1 : L;
+        double sp = 0.0;
+        for (int i = 0; i <= k2 + p; i++)
+            sp += ofact(k2 + p) * ofact(k1 + k2 - 1 - i) / ofact(k2 + p - i) / pow(g2, i + 1) /
+                  pow(g, k1 + k2 - i);
+        v += sgn == 0 ? sp : -sp;
+    }
+    return v;
+}
+
+double zvo_mono_exact(const int *nx, const int *ny, const int *nz, const int *nsto,
+                      const double *zeta)
+{
+    if (!g_ylm_ready) ylm_expansion();
+    int n[4], lm[4];
+    for (int k = 0; k < 4; k++) {
+        lm[k] = nx[k] + ny[k] + nz[k];
+        if (lm[k] > 2) return NAN; /* the reference stops: only s, p, d */
+        n[k] = 1 + lm[k] + nsto[k];
+    }
+    const int k1 = n[0] + n[1], k2 = n[2] + n[3];
+    const double g1 = zeta[0] + zeta[1], g2 = zeta[2] + zeta[3];
+    double complex sum = 0.0;
+    for (int l1 = 0; l1 <= lm[0]; l1++)
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int l2 = 0; l2 <= lm[1]; l2++)
+                for (int m2 = -l2; m2 <= l2; m2++)
+                    for (int l3 = 0; l3 <= lm[2]; l3++)
+                        for (int m3 = -l3; m3 <= l3; m3++)
+                            for (int l4 = 0; l4 <= lm[3]; l4++)
+                                for (int m4 = -l4; m4 <= l4; m4++) {
+                                    const double complex c =
+                                        conj(g_ylm_c[nx[0]][ny[0]][nz[0]][l1][m1 + 2]) *
+                                        g_ylm_c[nx[1]][ny[1]][nz[1]][l2][m2 + 2] *
+                                        conj(g_ylm_c[nx[2]][ny[2]][nz[2]][l3][m3 + 2]) *
+                                        g_ylm_c[nx[3]][ny[3]][nz[3]][l4][m4 + 2];
+                                    if (cabs(c) <= 1e-20) continue;
+                                    /* 1/r12 = sum_LM 4pi/(2L+1) r<^L/r>^(L+1) Y*_LM(1) Y_LM(2):
+                                     * int Y*_1 Y_2 Y*_LM = (-1)^M G(l1 L l2; m1 -M m2) needs
+                                     * M = m2 - m1; int Y*_3 Y_4 Y_LM = G(l3 L l4; m3 M m4)
+                                     * needs M = m3 - m4 */
+                                    const int M = m2 - m1;
+                                    if (m3 - m4 != M) continue;
+                                    double s = 0.0;
+                                    const int Lmax = l1 + l2 < l3 + l4 ? l1 + l2 : l3 + l4;
+                                    for (int L = abs(M); L <= Lmax; L++)
+                                        s += ((abs(M) & 1) ? -1.0 : 1.0) *
+                                             zvo_gaunt(l1, L, l2, m1, -M, m2) *
+                                             zvo_gaunt(l3, L, l4, m3, M, m4) *
+                                             radial_F(k1, g1, k2, g2, L) / (2.0 * L + 1.0);
+                                    sum += c * 4.0 * ZVO_PI * s;
+                                }
+    return creal(sum);
+}

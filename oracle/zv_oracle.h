@@ -57,6 +57,13 @@ double zvo_rint(int n, double rho);
 
 void zvo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 
+/* exact one-centre Slater integral (ik|jl) of x^nx y^ny z^nz r^nsto e^{-zeta r}
+ * factors ordered i, k, j, l (src/compute_int_slater.F90); NAN above d */
+double zvo_mono_exact(const int *nx, const int *ny, const int *nz, const int *nsto,
+                      const double *zeta);
+double zvo_gaunt(int l1, int l2, int l3, int m1, int m2, int m3);
+void zvo_ylm_coef(int a, int b, int c, int l, int m, double *re, double *im);
+
 #ifdef __cplusplus
 }
 #endif

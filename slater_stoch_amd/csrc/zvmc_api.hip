@@ -445,6 +445,15 @@ extern "C" int zv_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_
     return rc;
 }
 
+extern "C" int zv_exact_monocenter(zv_context *c, double *ijkl, double *err, int64_t *n_mono)
+{
+    ZV_NEED_BASIS(c);
+    if (!ijkl) return zv_fail(c, ZV_ERR_ARG, "null ijkl");
+    int rc = zv_run_exact_monocenter(c, ijkl, err, n_mono);
+    if (rc == ZV_OK) rc = zv_collect_timings(c);
+    return rc;
+}
+
 extern "C" int zv_gauss_ijkl_list(zv_context *c, int64_t nq, const int *qi, const int *qk,
                                   const int *qj, const int *ql, double *out)
 {

@@ -36,6 +36,7 @@ SIGNATURES = {
     "zv_nint": (I64, [I32]),
     "zv_build_mapping": (I32, [I32, VP, VP, VP, VP]),
     "zv_gauss_eri": (I32, [VP, VP, VP, VP]),
+    "zv_exact_monocenter": (I32, [VP, VP, VP, VP]),
     "zv_gauss_ijkl_list": (I32, [VP, I64, VP, VP, VP, VP, VP]),
     "zv_mc_blocks": (I32, [VP, I32, I64, I32, U64, U64, I32, VP, VP, VP, VP, VP, VP]),
     "zv_draw_samples": (I32, [VP, U64, U64, I64, I64, VP, VP, VP]),
@@ -144,6 +145,19 @@ class Engine:
         nscr = np.zeros(1, np.int64)
         self._check(self.lib.zv_gauss_eri(self.h, _p(out), _p(pre), _p(nscr)))
         return out, pre, int(nscr[0])
+
+    def exact_monocenter(self, ijkl, err=None):
+        """Overwrite the one-centre quartets of ijkl[nint] (kcp order) with
+        their exact Slater integrals and err[kcp] with 0, in place
+        (src/two_e_slater.F90:318-365); returns the number of one-centre
+        quartets."""
+        nint = self.basis.nint
+        assert ijkl.dtype == np.float64 and ijkl.flags.c_contiguous and ijkl.size == nint
+        if err is not None:
+            assert err.dtype == np.float64 and err.flags.c_contiguous and err.size == nint
+        nm = np.zeros(1, np.int64)
+        self._check(self.lib.zv_exact_monocenter(self.h, _p(ijkl), _p(err), _p(nm)))
+        return int(nm[0])
 
     def gauss_ijkl(self, qi, qk, qj, ql):
         qi, qk, qj, ql = [np.ascontiguousarray(a, np.int32) for a in (qi, qk, qj, ql)]

@@ -44,6 +44,8 @@ def load_ref():
     lib.ref_compare_nuclei.restype = ctypes.c_int
     lib.ref_compare_nuclei.argtypes = [ctypes.c_int] * 4
     lib.ref_build_mapping.restype = ctypes.c_int64
+    lib.ref_int_slater.restype = ctypes.c_double
+    lib.ref_int_slater.argtypes = [ctypes.c_int] * 4
     lib.ref_get_basis.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 10
     lib.ref_mc_fixed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 6
@@ -201,5 +203,35 @@ def main():
     print("done")
 
 
+def mono():
+    """Exact one-centre Slater integrals (src/compute_int_slater.F90) of every
+    one-centre quartet of a three-atom molecule whose atoms carry every shell
+    type the reference's exact formulas accept (s, p, d with 1S-3S, 2P-3P,
+    3D) -> mono_exact.npz."""
+    lib = load_ref()
+    geom = B.geometry_from_angstrom(["O", "H", "N"],
+                                    [[0.1, -0.2, 0.05], [0.3, 1.05, -0.4], [-1.1, 0.2, 0.7]])
+    basis = {"O": [("1S", 7.7), ("2S", 2.2), ("3S", 1.4), ("2P", 2.3), ("3P", 1.1), ("3D", 1.9)],
+             "H": [("1S", 1.3), ("2P", 1.6)],
+             "N": [("1S", 6.6), ("2S", 1.9), ("2P", 1.8), ("3D", 2.4)]}
+    bas = B.expand_basis(geom, basis, ng0=4)
+    n = setup(lib, bas, "mono", 4)
+    rb = get_basis(lib, n, bas.ldg)
+    q = mapping(lib, n)
+    nuc = bas.nucleus
+    one = np.flatnonzero((nuc[q[0] - 1] == nuc[q[1] - 1]) & (nuc[q[0] - 1] == nuc[q[2] - 1])
+                         & (nuc[q[0] - 1] == nuc[q[3] - 1]))
+    print(f"mono: {n} AOs, {q[0].size} quartets, {one.size} one-centre", flush=True)
+    vals = np.array([lib.ref_int_slater(int(q[0][t]), int(q[2][t]), int(q[1][t]), int(q[3][t]))
+                     for t in one])
+    np.savez(OUT / "mono_exact.npz", kcp=one, exact=vals, is_=q[0][one], js=q[1][one],
+             ks=q[2][one], ls=q[3][one], **rb)
+    print("done")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["mono"]:
+        mono()
+    else:
+        main()
+        mono()

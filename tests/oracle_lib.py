@@ -47,6 +47,11 @@ def _load():
     lib.zvo_rint_sum.restype = ctypes.c_double
     lib.zvo_rint_sum.argtypes = [ctypes.c_int, ctypes.c_double, VP]
     lib.zvo_philox4x32_10.argtypes = [VP, VP, VP]
+    lib.zvo_mono_exact.restype = ctypes.c_double
+    lib.zvo_mono_exact.argtypes = [VP] * 5
+    lib.zvo_gaunt.restype = ctypes.c_double
+    lib.zvo_gaunt.argtypes = [ctypes.c_int] * 6
+    lib.zvo_ylm_coef.argtypes = [ctypes.c_int] * 5 + [VP, VP]
     return lib
 
 
@@ -119,6 +124,18 @@ class Oracle:
 
     def rint(self, n, rho):
         return self.lib.zvo_rint(int(n), float(rho))
+
+    def mono_exact(self, i, k, j, l):
+        """Exact one-centre Slater integral (ik|jl) of the loaded basis
+        (1-based AO indices; the AOs are assumed to share one centre)."""
+        b = self.basis
+        idx = [int(i) - 1, int(k) - 1, int(j) - 1, int(l) - 1]
+        nx = np.array([b.npower[a, 0] for a in idx], np.int32)
+        ny = np.array([b.npower[a, 1] for a in idx], np.int32)
+        nz = np.array([b.npower[a, 2] for a in idx], np.int32)
+        ns = np.array([b.n_sto[a] for a in idx], np.int32)
+        z = np.array([b.zeta[a] for a in idx], np.float64)
+        return self.lib.zvo_mono_exact(P(nx), P(ny), P(nz), P(ns), P(z))
 
     def philox(self, ctr, key):
         c = np.array(ctr, np.uint32)

@@ -1,6 +1,7 @@
 """CPU checks: the host input layer and the oracle against the reference's
 golden vectors (tests/golden/, written from the compiled reference by
 make_golden.py) and against closed forms / invariants of the mathematics."""
+import ctypes
 import math
 
 import numpy as np
@@ -310,3 +311,67 @@ def test_reference_inverse_erfc_is_erfcinv():
     any accurate inverse erfc (src/draw_configuration.F90:8-25)."""
     g = golden("scalars.npz")
     np.testing.assert_allclose(g["dierfc"], special.erfcinv(g["y"]), rtol=1e-13, atol=1e-15)
+
+
+def mono_basis():
+    """The three-atom basis of tests/golden/mono_exact.npz (every shell type
+    the reference's exact one-centre formulas accept: 1S-3S, 2P-3P, 3D)."""
+    geom = B.geometry_from_angstrom(["O", "H", "N"],
+                                    [[0.1, -0.2, 0.05], [0.3, 1.05, -0.4], [-1.1, 0.2, 0.7]])
+    basis = {"O": [("1S", 7.7), ("2S", 2.2), ("3S", 1.4), ("2P", 2.3), ("3P", 1.1), ("3D", 1.9)],
+             "H": [("1S", 1.3), ("2P", 1.6)],
+             "N": [("1S", 6.6), ("2S", 1.9), ("2P", 1.8), ("3D", 2.4)]}
+    return B.expand_basis(geom, basis, ng0=4)
+
+
+def test_mono_exact_matches_reference(oracle):
+    """All 9526 one-centre quartets of the mono basis against the reference
+    build's compute_int_slater (src/compute_int_slater.F90)."""
+    g = golden("mono_exact.npz")
+    bas = mono_basis()
+    np.testing.assert_array_equal(bas.zeta, g["zeta"])
+    np.testing.assert_array_equal(bas.n_sto, g["n_sto"])
+    oracle.set_basis(bas)
+    got = np.array([oracle.mono_exact(i, k, j, l)
+                    for i, k, j, l in zip(g["is_"], g["ks"], g["js"], g["ls"])])
+    ref = g["exact"]
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * scale)
+    assert (ref == 0).sum() > 1000 and (ref != 0).sum() > 1000
+
+
+def test_mono_exact_closed_forms(oracle):
+    """(1s1s|1s1s) = (pi/z^3)^2 5z/8 for unnormalized 1s (the textbook 5z/8
+    for normalized ones); the Gaunt coefficients equal a direct quadrature of
+    int Y*_l1m1 Y_l2m2 Y_l3m3 over the sphere; the monomial expansion
+    reproduces x^2/r^2 pointwise."""
+    bas = configs.water_minimal(ng0=4)
+    oracle.set_basis(bas)
+    z = bas.zeta[0]
+    assert math.isclose(oracle.mono_exact(1, 1, 1, 1), (math.pi / z ** 3) ** 2 * 5 * z / 8,
+                        rel_tol=1e-13)
+    lib = oracle.lib
+    # product Gauss-Legendre (cos theta) x trapezoid (phi) is exact here
+    xg, wg = np.polynomial.legendre.leggauss(12)
+    phis = 2 * np.pi * np.arange(24) / 24
+
+    def ylm(l, m, ct, ph):
+        return special.sph_harm_y(l, m, np.arccos(ct), ph) if hasattr(special, "sph_harm_y") \
+            else special.sph_harm(m, l, ph, np.arccos(ct))
+
+    for (l1, l2, l3, m1, m2, m3) in [(0, 0, 0, 0, 0, 0), (1, 1, 0, 1, 1, 0), (2, 1, 1, 0, 1, -1),
+                                    (2, 2, 2, 1, -1, 2), (1, 2, 1, -1, 0, -1), (2, 2, 0, -2, -2, 0),
+                                    (3, 2, 1, 1, 0, 1), (4, 2, 2, 2, 1, 1)]:
+        num = sum(w * (2 * np.pi / 24) * np.conj(ylm(l1, m1, c, p)) * ylm(l2, m2, c, p) *
+                  ylm(l3, m3, c, p) for c, w in zip(xg, wg) for p in phis)
+        assert abs(lib.zvo_gaunt(l1, l2, l3, m1, m2, m3) - num.real) < 1e-13
+        assert abs(num.imag) < 1e-13
+    re, im = ctypes.c_double(), ctypes.c_double()
+    ct, ph = 0.3, 1.1
+    x = math.sqrt(1 - ct * ct) * math.cos(ph)
+    s = 0.0
+    for l in range(3):
+        for m in range(-l, l + 1):
+            lib.zvo_ylm_coef(2, 0, 0, l, m, ctypes.byref(re), ctypes.byref(im))
+            s += complex(re.value, im.value) * ylm(l, m, ct, ph)
+    assert abs(s - x * x) < 1e-13
