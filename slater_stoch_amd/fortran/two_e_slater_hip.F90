@@ -11,6 +11,9 @@
 !                     executable)
 !   ZVMC_PRECISION    fp64_parity | fp64 (default) | fp32   (estimator)
 !   ZVMC_SVD_CLEAN    0 to skip the cleanup (default: run it, as the reference)
+!   ZVMC_EXACT_MONOCENTER  1 for the reference's do_exact_monocenter branch
+!                     (src/two_e_slater.F90:74,318-365: exact one-centre
+!                     integrals, standard error 0); default off, as there
 program two_e_slater_hip
   use iso_c_binding
   use zvmc
@@ -43,7 +46,8 @@ program two_e_slater_hip
   ! quartets and results
   integer(c_int64_t) :: nint, kcp, nscr
   integer(c_int), allocatable :: is(:), js(:), ks(:), ls(:)
-  double precision, allocatable :: ijkl_gaus(:), precond(:), mean(:), err(:)
+  double precision, allocatable :: ijkl_gaus(:), precond(:), mean(:), err(:), exact(:)
+  integer(c_int64_t) :: nmono
   double precision :: error_max, errmoy, dmoy, enucl, rij
   type(c_ptr) :: ctx
   integer(c_int) :: rc, prec, nrem, rank
@@ -182,6 +186,23 @@ program two_e_slater_hip
   call zv_check(ctx, zv_mc_blocks(ctx, nbl, int(npts,c_int64_t), ZV_SAMPLES_PHILOX, seed, &
       0_c_int64_t, prec, c_null_ptr, c_null_ptr, c_null_ptr, mean, err, c_null_ptr), &
       'zv_mc_blocks')
+
+  ! exact one-centre quartets: value exact, error 0, and no Gaussian-fit part
+  ! in the final sum (src/two_e_slater.F90:359-361, 489-493, 558-563)
+  call get_environment_variable('ZVMC_EXACT_MONOCENTER', env)
+  if (trim(env) == '1') then
+    allocate(exact(nint))
+    exact(:) = -huge(1.d0)
+    call zv_check(ctx, zv_exact_monocenter(ctx, exact, err, nmono), 'zv_exact_monocenter')
+    print *, 'One-center integrals computed exactly: ', nmono
+    do kcp = 1, nint
+      if (exact(kcp) /= -huge(1.d0)) then
+        mean(kcp) = exact(kcp)
+        ijkl_gaus(kcp) = 0.d0
+      end if
+    end do
+    deallocate(exact)
+  end if
 
   ! report as src/two_e_slater.F90:524-553
   error_max = -1.d0

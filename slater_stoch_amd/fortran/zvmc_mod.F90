@@ -61,6 +61,14 @@ module zvmc
       integer(c_int64_t), intent(out) :: n_screened
     end function
 
+    integer(c_int) function zv_exact_monocenter(ctx, ijkl, err, n_mono) &
+        bind(C, name="zv_exact_monocenter")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: ijkl(*), err(*)
+      integer(c_int64_t), intent(out) :: n_mono
+    end function
+
     integer(c_int) function zv_gauss_ijkl_list(ctx, nq, qi, qk, qj, ql, out) &
         bind(C, name="zv_gauss_ijkl_list")
       import :: c_int, c_ptr, c_double, c_int64_t

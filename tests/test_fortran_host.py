@@ -72,3 +72,27 @@ def test_driver_with_cleanup_runs(tmp_path):
     assert p.returncode == 0, p.stdout + p.stderr
     assert "Removed" in p.stdout
     assert len((tmp_path / "bielec_ao").read_text().splitlines()) == bas.nint
+
+
+@pytest.mark.gpu
+def test_exact_monocenter_branch(tmp_path):
+    """ZVMC_EXACT_MONOCENTER=1 (the reference's do_exact_monocenter): the
+    one-centre quartets carry the exact Slater integral and error 0, the
+    others the Gaussian-fit ERI plus the Monte Carlo mean as before."""
+    bas, p = _run(tmp_path, env={"ZVMC_PRECISION": "fp64_parity", "ZVMC_SVD_CLEAN": "0",
+                                 "ZVMC_EXACT_MONOCENTER": "1"})
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert int(p.stdout.split("One-center integrals computed exactly:")[1].split()[0]) == 122
+    lines = (tmp_path / "bielec_ao").read_text().splitlines()
+    val = np.array([float(ln[24:46].replace("D", "E")) for ln in lines])
+    err = np.array([float(ln[46:68].replace("D", "E")) for ln in lines])
+    from slater_stoch_amd.zvmc import Engine
+    with Engine(0) as eng:
+        eng.set_basis(bas)
+        eri, _, _ = eng.gauss_eri()
+        mean, stderr = eng.mc_blocks(10, 400, "fp64_parity", seed=1, stream=0)
+        want, werr = eri + mean, stderr.copy()
+        nm = eng.exact_monocenter(want, werr)
+    assert nm == 122 and (err == 0).sum() >= 122
+    np.testing.assert_allclose(val, want, rtol=1e-14, atol=1e-300)
+    np.testing.assert_allclose(err, werr, rtol=1e-14, atol=1e-300)
