@@ -89,6 +89,16 @@ int zv_gauss_eri(zv_context *ctx, double *ijkl_gaus, double *precond, int64_t *n
  * ZV_ERR_ARG, where the reference stops. */
 int zv_exact_monocenter(zv_context *ctx, double *ijkl, double *err, int64_t *n_mono);
 
+/* One-electron integrals over the Gaussian fits of the loaded basis, the
+ * loop of src/one_e_slater.F90:87-95 over src/one_elect.F90: overlap S,
+ * nuclear attraction V (src/one_electron_Vij.F90: -sum over nuclei of
+ * charge x NAI) and kinetic K (src/one_electron_Kij.F90), each n x n
+ * column-major (out; S(i,k) at S[(i-1) + n (k-1)]).  Nuclei: charge[nnuc]
+ * and nuc[3 c + d] in bohr (src/read_geometry.F90: charge = atomic number).
+ * AOs above degree 6 (with the r^2 of i_type 2) give ZV_ERR_ARG. */
+int zv_one_e(zv_context *ctx, int nnuc, const double *charge, const double *nuc, double *S,
+             double *V, double *K);
+
 /* gauss_ijkl(i,k,j,l) for an explicit list of nq quartets (1-based orbital
  * indices; no screening).  src/gauss_ijkl.F90:51 batched. */
 int zv_gauss_ijkl_list(zv_context *ctx, int64_t nq, const int *qi, const int *qk,

@@ -141,6 +141,24 @@ contains
     ref_int_slater = v
   end function
 
+  ! Overlap, nuclear attraction and kinetic integrals of the loaded basis
+  ! over the Gaussian fits, S(i,k), V(i,k), K(i,k) column-major n x n
+  ! (src/one_elect.F90, the loop of src/one_e_slater.F90:87-95).
+  subroutine ref_one_e(s_out, v_out, k_out) bind(C, name="ref_one_e")
+    real(c_double), intent(out) :: s_out(nbasis, nbasis), v_out(nbasis, nbasis), &
+                                   k_out(nbasis, nbasis)
+    integer :: i, k
+    double precision :: sij, vij, kij
+    do i = 1, nbasis
+      do k = 1, nbasis
+        call one_elect(i, k, sij, vij, kij)
+        s_out(i, k) = sij
+        v_out(i, k) = vij
+        k_out(i, k) = kij
+      end do
+    end do
+  end subroutine
+
   real(c_double) function ref_rint(n, rho) bind(C, name="ref_rint")
     integer(c_int), value :: n
     real(c_double), value :: rho

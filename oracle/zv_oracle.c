@@ -1055,3 +1055,149 @@ double zvo_mono_exact(const int *nx, const int *ny, const int *nz, const int *ns
                                 }
     return creal(sum);
 }
+
+/* ------------------------------------------------------------------ */
+/* One-electron integrals over the Gaussian fits: overlap S, nuclear   */
+/* attraction V, kinetic K of every ordered AO pair (src/one_e_slater.F90 */
+/* :87-95, src/one_elect.F90, src/one_electron_sij.F90,                 */
+/* src/one_electron_Kij.F90, src/one_electron_Vij.F90,                  */
+/* src/NAI_pol_mult.F90).  Per primitive pair (McMurchie-Davidson):     */
+/* S = e^{-mu AB^2} prod_d E^{a_d b_d}_0 sqrt(pi/p), zero when           */
+/* mu AB^2 > -ln 1e-15 (src/one_electron_sij.F90:44-46); K = -1/2 of the */
+/* bra Laplacian in the reference's form (src/one_electron_Kij.F90);    */
+/* V = -sum_C Z_C 2pi/p e^{-mu AB^2} sum_tuv E_t E_u E_v R_tuv(p, P-C),  */
+/* unscreened as NAI_pol_mult.                                          */
+/* ------------------------------------------------------------------ */
+#define OE_MAXA 8  /* degree of a bra component + 2 */
+#define OE_MAXT 17
+
+/* e[i][t] = E^{i,b}_t for i = 0..imax (1D Hermite expansion coefficients
+ * without the e^{-mu X^2} factor) */
+static void oe_hermite(int imax, int b, double p, double xpa, double xpb,
+                       double e[OE_MAXA + 1][OE_MAXT])
+{
+    double cur[OE_MAXT] = {0}, nxt[OE_MAXT];
+    cur[0] = 1.0;
+    const double h = 0.5 / p;
+    for (int j = 0; j < b; j++) {
+        for (int t = 0; t <= j + 1; t++)
+            nxt[t] = (t > 0 ? h * cur[t - 1] : 0.0) + xpb * cur[t] + (t + 1) * cur[t + 1];
+        for (int t = 0; t <= j + 1; t++) cur[t] = nxt[t];
+        cur[j + 2] = 0.0;
+    }
+    for (int t = 0; t < OE_MAXT; t++) e[0][t] = t <= b ? cur[t] : 0.0;
+    for (int i = 0; i < imax; i++) {
+        for (int t = 0; t < OE_MAXT; t++) e[i + 1][t] = 0.0;
+        for (int t = 0; t <= i + 1 + b; t++)
+            e[i + 1][t] = (t > 0 ? h * e[i][t - 1] : 0.0) + xpa * e[i][t] +
+                          (t + 1 < OE_MAXT ? (t + 1) * e[i][t + 1] : 0.0);
+    }
+}
+
+/* R_tuv(alpha; X, Y, Z) for t + u + v <= L, r[t][u][v] */
+static void oe_hermite_R(int L, double alpha, double X, double Y, double Z,
+                         double r[OE_MAXT][OE_MAXT][OE_MAXT])
+{
+    static double w[OE_MAXT][OE_MAXT][OE_MAXT][OE_MAXT];
+    const double T = alpha * (X * X + Y * Y + Z * Z);
+    for (int n = L; n >= 0; n--) {
+        double pw = 1.0;
+        for (int q = 0; q < n; q++) pw *= -2.0 * alpha;
+        w[n][0][0][0] = pw * zvo_rint(n, T);
+        for (int s = 1; s <= L - n; s++)
+            for (int t = s; t >= 0; t--)
+                for (int u = s - t; u >= 0; u--) {
+                    const int v = s - t - u;
+                    double val;
+                    if (t > 0)
+                        val = X * w[n + 1][t - 1][u][v] + (t > 1 ? (t - 1) * w[n + 1][t - 2][u][v] : 0.0);
+                    else if (u > 0)
+                        val = Y * w[n + 1][t][u - 1][v] + (u > 1 ? (u - 1) * w[n + 1][t][u - 2][v] : 0.0);
+                    else
+                        val = Z * w[n + 1][t][u][v - 1] + (v > 1 ? (v - 1) * w[n + 1][t][u][v - 2] : 0.0);
+                    w[n][t][u][v] = val;
+                }
+    }
+    for (int t = 0; t <= L; t++)
+        for (int u = 0; t + u <= L; u++)
+            for (int v = 0; t + u + v <= L; v++) r[t][u][v] = w[0][t][u][v];
+}
+
+/* the Gaussian terms of AO o: nc terms of coefficients cf[term][m] and powers pw[term][3]
+ * (src/one_elect.F90:20-72) */
+static int oe_terms(int o, const double **cf, int pw[4][3])
+{
+    const int t = g_itype[o];
+    const int *np = g_npower + 3 * o;
+    const double *c1 = g_c1 + (size_t)o * g_ldg, *c2 = g_c2 + (size_t)o * g_ldg;
+    int nc = t == 1 ? 1 : (t == 2 ? 3 : 4);
+    for (int ii = 0; ii < nc; ii++) {
+        const int shift = t == 2 ? ii : ii - 1; /* which coordinate carries +2, -1: none */
+        cf[ii] = (t == 1 || (t == 3 && ii == 0)) ? c1 : c2;
+        for (int d = 0; d < 3; d++) pw[ii][d] = np[d] + (d == shift ? 2 : 0);
+    }
+    return nc;
+}
+
+void zvo_one_e(int nnuc, const double *charge, const double *nuc, double *S, double *V, double *K)
+{
+    const double thr = -log(1.0e-15);
+    static double r[OE_MAXT][OE_MAXT][OE_MAXT];
+    for (int i = 0; i < g_n; i++)
+        for (int k = 0; k < g_n; k++) {
+            const double *ci[4], *ck[4];
+            int pi[4][3], pk[4][3];
+            const int ni = oe_terms(i, ci, pi), nk = oe_terms(k, ck, pk);
+            const double *A = g_center + 3 * i, *Bc = g_center + 3 * k;
+            double s = 0.0, v = 0.0, kin = 0.0;
+            for (int a = 0; a < g_ngauss[i]; a++)
+                for (int b = 0; b < g_ngauss[k]; b++) {
+                    const double al = g_gg[(size_t)i * g_ldg + a], be = g_gg[(size_t)k * g_ldg + b];
+                    const double p = al + be, mu = al * be / p;
+                    double P[3], ab2 = 0.0;
+                    for (int d = 0; d < 3; d++) {
+                        P[d] = (al * A[d] + be * Bc[d]) / p;
+                        ab2 += (A[d] - Bc[d]) * (A[d] - Bc[d]);
+                    }
+                    const double arg = mu * ab2, kab = exp(-arg);
+                    for (int ii = 0; ii < ni; ii++)
+                        for (int jj = 0; jj < nk; jj++) {
+                            const double dd = ci[ii][a] * ck[jj][b];
+                            double e[3][OE_MAXA + 1][OE_MAXT];
+                            for (int d = 0; d < 3; d++)
+                                oe_hermite(pi[ii][d] + 2, pk[jj][d], p, P[d] - A[d], P[d] - Bc[d],
+                                           e[d]);
+                            const double sq = sqrt(ZVO_PI / p);
+                            if (!(arg > thr)) {
+                                double s1[3], lap = 0.0;
+                                for (int d = 0; d < 3; d++) s1[d] = e[d][pi[ii][d]][0] * sq;
+                                for (int d = 0; d < 3; d++) {
+                                    const int na = pi[ii][d];
+                                    double d2 = -2.0 * al * (2 * na + 1) * e[d][na][0] * sq +
+                                                4.0 * al * al * e[d][na + 2][0] * sq;
+                                    if (na >= 2) d2 += na * (na - 1.0) * e[d][na - 2][0] * sq;
+                                    lap += d2 * s1[(d + 1) % 3] * s1[(d + 2) % 3];
+                                }
+                                s += dd * kab * s1[0] * s1[1] * s1[2];
+                                kin += dd * kab * (-0.5) * lap;
+                            }
+                            const int L = pi[ii][0] + pi[ii][1] + pi[ii][2] + pk[jj][0] +
+                                          pk[jj][1] + pk[jj][2];
+                            for (int cn = 0; cn < nnuc; cn++) {
+                                oe_hermite_R(L, p, P[0] - nuc[3 * cn], P[1] - nuc[3 * cn + 1],
+                                             P[2] - nuc[3 * cn + 2], r);
+                                double acc = 0.0;
+                                for (int t = 0; t <= pi[ii][0] + pk[jj][0]; t++)
+                                    for (int u = 0; u <= pi[ii][1] + pk[jj][1]; u++)
+                                        for (int w = 0; w <= pi[ii][2] + pk[jj][2]; w++)
+                                            acc += e[0][pi[ii][0]][t] * e[1][pi[ii][1]][u] *
+                                                   e[2][pi[ii][2]][w] * r[t][u][w];
+                                v -= dd * charge[cn] * (2.0 * ZVO_PI / p) * kab * acc;
+                            }
+                        }
+                }
+            S[i + (size_t)k * g_n] = s;
+            V[i + (size_t)k * g_n] = v;
+            K[i + (size_t)k * g_n] = kin;
+        }
+}

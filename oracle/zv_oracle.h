@@ -64,6 +64,12 @@ double zvo_mono_exact(const int *nx, const int *ny, const int *nz, const int *ns
 double zvo_gaunt(int l1, int l2, int l3, int m1, int m2, int m3);
 void zvo_ylm_coef(int a, int b, int c, int l, int m, double *re, double *im);
 
+/* overlap, nuclear attraction and kinetic integrals over the Gaussian fits
+ * of the loaded basis, n x n column-major (src/one_elect.F90); nuclei:
+ * charge[nnuc], nuc[3*c+d] */
+void zvo_one_e(int nnuc, const double *charge, const double *nuc, double *S, double *V,
+               double *K);
+
 #ifdef __cplusplus
 }
 #endif

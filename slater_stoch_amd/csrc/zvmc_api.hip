@@ -454,6 +454,17 @@ extern "C" int zv_exact_monocenter(zv_context *c, double *ijkl, double *err, int
     return rc;
 }
 
+extern "C" int zv_one_e(zv_context *c, int nnuc, const double *charge, const double *nuc,
+                        double *S, double *V, double *K)
+{
+    ZV_NEED_BASIS(c);
+    if (nnuc < 1 || !charge || !nuc) return zv_fail(c, ZV_ERR_ARG, "no nuclei");
+    if (!S || !V || !K) return zv_fail(c, ZV_ERR_ARG, "null output");
+    int rc = zv_run_one_e(c, nnuc, charge, nuc, S, V, K);
+    if (rc == ZV_OK) rc = zv_collect_timings(c);
+    return rc;
+}
+
 extern "C" int zv_gauss_ijkl_list(zv_context *c, int64_t nq, const int *qi, const int *qk,
                                   const int *qj, const int *ql, double *out)
 {

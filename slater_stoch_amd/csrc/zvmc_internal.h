@@ -168,6 +168,8 @@ int zv_run_pair_samples(zv_context *c, int i, int k, int ns, const double *r1, c
                         double *rhog2, double *jac1, double *jac2);
 int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_screened);
 int zv_run_exact_monocenter(zv_context *c, double *ijkl, double *err, int64_t *n_mono);
+int zv_run_one_e(zv_context *c, int nnuc, const double *charge, const double *nuc, double *S,
+                 double *V, double *K);
 int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, const int *qj,
                       const int *ql, double *out);
 int zv_run_scatter_ref(zv_context *c, const double *d_tri, double *host_out);

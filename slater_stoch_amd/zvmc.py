@@ -37,6 +37,7 @@ SIGNATURES = {
     "zv_build_mapping": (I32, [I32, VP, VP, VP, VP]),
     "zv_gauss_eri": (I32, [VP, VP, VP, VP]),
     "zv_exact_monocenter": (I32, [VP, VP, VP, VP]),
+    "zv_one_e": (I32, [VP, I32, VP, VP, VP, VP, VP]),
     "zv_gauss_ijkl_list": (I32, [VP, I64, VP, VP, VP, VP, VP]),
     "zv_mc_blocks": (I32, [VP, I32, I64, I32, U64, U64, I32, VP, VP, VP, VP, VP, VP]),
     "zv_draw_samples": (I32, [VP, U64, U64, I64, I64, VP, VP, VP]),
@@ -158,6 +159,24 @@ class Engine:
         nm = np.zeros(1, np.int64)
         self._check(self.lib.zv_exact_monocenter(self.h, _p(ijkl), _p(err), _p(nm)))
         return int(nm[0])
+
+    def one_e(self, charges=None, nuclei=None):
+        """Overlap, nuclear attraction and kinetic integrals over the Gaussian
+        fits (src/one_elect.F90): (S, V, K) as n x n arrays, [i-1, k-1] =
+        (i|op|k).  Nuclei default to the basis geometry (charge = atomic
+        number, src/read_geometry.F90:23)."""
+        from . import basis as _B
+        geo = self.basis.geometry
+        if charges is None:
+            charges = [float(_B.number_atom(lab)) for lab in geo.labels]
+        if nuclei is None:
+            nuclei = geo.coords
+        q = np.ascontiguousarray(charges, np.float64)
+        c = np.ascontiguousarray(nuclei, np.float64).reshape(-1, 3)
+        n = self.basis.n
+        S, V, K = (np.zeros((n, n), order="F") for _ in range(3))
+        self._check(self.lib.zv_one_e(self.h, q.size, _p(q), _p(c), _p(S), _p(V), _p(K)))
+        return S, V, K
 
     def gauss_ijkl(self, qi, qk, qj, ql):
         qi, qk, qj, ql = [np.ascontiguousarray(a, np.int32) for a in (qi, qk, qj, ql)]

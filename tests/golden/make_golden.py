@@ -44,6 +44,7 @@ def load_ref():
     lib.ref_compare_nuclei.restype = ctypes.c_int
     lib.ref_compare_nuclei.argtypes = [ctypes.c_int] * 4
     lib.ref_build_mapping.restype = ctypes.c_int64
+    lib.ref_one_e.argtypes = [ctypes.c_void_p] * 3
     lib.ref_int_slater.restype = ctypes.c_double
     lib.ref_int_slater.argtypes = [ctypes.c_int] * 4
     lib.ref_get_basis.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 10
@@ -229,9 +230,36 @@ def mono():
     print("done")
 
 
+def one_e():
+    """Overlap, nuclear attraction and kinetic integrals over the Gaussian
+    fits (src/one_elect.F90, as src/one_e_slater.F90:87-95 loops them) for
+    water (20 Gaussians) and the mixed basis with 3S-5G (4 Gaussians)
+    -> one_e.npz."""
+    lib = load_ref()
+    out = {}
+    water = configs.water_minimal(ng0=20)
+    geom = B.geometry_from_angstrom(["O", "H", "N"],
+                                    [[0.1, -0.2, 0.05], [0.3, 1.05, -0.4], [-1.1, 0.2, 0.7]])
+    mixed = B.expand_basis(geom, {"O": [("1S", 7.7), ("2S", 2.2), ("3S", 1.4), ("2P", 2.3),
+                                        ("3P", 1.1), ("3D", 1.9)],
+                                  "H": [("1S", 1.3), ("2P", 1.6)],
+                                  "N": [("4F", 1.7), ("5G", 2.1)]}, ng0=4)
+    for tag, bas, ng in (("water", water, 20), ("mixed", mixed, 4)):
+        n = setup(lib, bas, f"one_e_{tag}", ng)
+        S, V, K = (np.zeros((n, n), order="F") for _ in range(3))
+        lib.ref_one_e(P(S), P(V), P(K))
+        out.update({f"{tag}_S": S, f"{tag}_V": V, f"{tag}_K": K, f"{tag}_zeta": bas.zeta})
+        print(f"one_e {tag}: {n} AOs", flush=True)
+    np.savez(OUT / "one_e.npz", **out)
+    print("done")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["mono"]:
         mono()
+    elif sys.argv[1:] == ["one_e"]:
+        one_e()
     else:
         main()
         mono()
+        one_e()

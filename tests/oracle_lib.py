@@ -52,6 +52,7 @@ def _load():
     lib.zvo_gaunt.restype = ctypes.c_double
     lib.zvo_gaunt.argtypes = [ctypes.c_int] * 6
     lib.zvo_ylm_coef.argtypes = [ctypes.c_int] * 5 + [VP, VP]
+    lib.zvo_one_e.argtypes = [ctypes.c_int] + [VP] * 5
     return lib
 
 
@@ -136,6 +137,18 @@ class Oracle:
         ns = np.array([b.n_sto[a] for a in idx], np.int32)
         z = np.array([b.zeta[a] for a in idx], np.float64)
         return self.lib.zvo_mono_exact(P(nx), P(ny), P(nz), P(ns), P(z))
+
+    def one_e(self):
+        """(S, V, K) over the Gaussian fits of the loaded basis, nuclei of its
+        geometry (charge = atomic number)."""
+        from slater_stoch_amd import basis as B
+        geo = self.basis.geometry
+        q = np.array([float(B.number_atom(lab)) for lab in geo.labels])
+        c = np.ascontiguousarray(np.asarray(geo.coords, np.float64))
+        n = self.basis.n
+        S, V, K = (np.zeros((n, n), order="F") for _ in range(3))
+        self.lib.zvo_one_e(q.size, P(q), P(c), P(S), P(V), P(K))
+        return S, V, K
 
     def philox(self, ctr, key):
         c = np.array(ctr, np.uint32)

@@ -375,3 +375,80 @@ def test_mono_exact_closed_forms(oracle):
             lib.zvo_ylm_coef(2, 0, 0, l, m, ctypes.byref(re), ctypes.byref(im))
             s += complex(re.value, im.value) * ylm(l, m, ct, ph)
     assert abs(s - x * x) < 1e-13
+
+
+def one_e_mixed_basis():
+    """The basis of tests/golden/one_e.npz 'mixed' (3S-5G, 4 Gaussians)."""
+    geom = B.geometry_from_angstrom(["O", "H", "N"],
+                                    [[0.1, -0.2, 0.05], [0.3, 1.05, -0.4], [-1.1, 0.2, 0.7]])
+    return B.expand_basis(geom, {"O": [("1S", 7.7), ("2S", 2.2), ("3S", 1.4), ("2P", 2.3),
+                                       ("3P", 1.1), ("3D", 1.9)],
+                                 "H": [("1S", 1.3), ("2P", 1.6)],
+                                 "N": [("4F", 1.7), ("5G", 2.1)]}, ng0=4)
+
+
+# The reference's nuclear attraction (src/NAI_pol_mult.F90:51) drops
+# polynomial coefficients below 1e-9 in absolute value; the McMurchie-Davidson
+# evaluation keeps them, which moves V by up to 3e-8 of its largest entry on
+# water (1.7e-7 relative on the 2S diagonal, where the closed form below sides
+# with the full evaluation).  S and K have no such cut.
+ONE_E_TOL = {"S": 1e-13, "K": 1e-13, "V": 5e-7}
+
+
+@pytest.mark.parametrize("tag", ["water", "mixed"])
+def test_one_e_matches_reference(oracle, tag):
+    """Overlap, nuclear attraction and kinetic matrices over the fits
+    (src/one_elect.F90) against the reference build, every AO pair."""
+    g = golden("one_e.npz")
+    bas = configs.water_minimal(ng0=20) if tag == "water" else one_e_mixed_basis()
+    np.testing.assert_array_equal(bas.zeta, g[f"{tag}_zeta"])
+    oracle.set_basis(bas)
+    got = dict(zip("SVK", oracle.one_e()))
+    for name in "SVK":
+        ref = g[f"{tag}_{name}"]
+        scale = np.abs(ref).max()
+        np.testing.assert_allclose(got[name], ref, rtol=0, atol=ONE_E_TOL[name] * scale,
+                                   err_msg=f"{tag} {name}")
+
+
+def test_one_e_closed_forms(oracle):
+    """On one centre: the s-s nuclear attraction is -sum Z c c 2pi/p F0(p R^2)
+    (R the distance to each nucleus), the r^2-type (2S) diagonal's own-nucleus
+    part is -Z sum c c 4pi/p^3, the s-s overlap sum c c (pi/p)^1.5 and the
+    s-s kinetic sum c c 3 a b/p (pi/p)^1.5."""
+    bas = configs.water_minimal(ng0=20)
+    oracle.set_basis(bas)
+    S, V, K = oracle.one_e()
+    Z = [float(B.number_atom(lab)) for lab in bas.geometry.labels]
+    C = np.asarray(bas.geometry.coords)
+
+    def f0(t):
+        return 1.0 if t < 1e-15 else 0.5 * math.sqrt(math.pi / t) * math.erf(math.sqrt(t))
+
+    for i in range(bas.n):
+        if bas.i_type[i] != 1 or bas.npower[i].sum():
+            continue
+        ng = bas.n_gauss[i]
+        c, g = bas.c_gauss[0, i, :ng], bas.g_gauss[i, :ng]
+        p = g[:, None] + g[None, :]
+        cc = c[:, None] * c[None, :]
+        v = sum(-zc * float(np.sum(cc * 2 * np.pi / p * np.vectorize(f0)(
+            p * np.sum((bas.center[i] - rc) ** 2)))) for zc, rc in zip(Z, C))
+        assert math.isclose(V[i, i], v, rel_tol=1e-13)
+        assert math.isclose(S[i, i], float(np.sum(cc * (np.pi / p) ** 1.5)), rel_tol=1e-13)
+        ab = g[:, None] * g[None, :]
+        assert math.isclose(K[i, i], float(np.sum(cc * 3 * ab / p * (np.pi / p) ** 1.5)),
+                            rel_tol=1e-12)
+    lib = oracle.lib
+    i = 1  # O 2S, i_type 2: c_gauss(:,2) r^2 e^{-g r^2}
+    ng = bas.n_gauss[i]
+    c, g = bas.c_gauss[1, i, :ng], bas.g_gauss[i, :ng]
+    p = g[:, None] + g[None, :]
+    own = -8.0 * float(np.sum(c[:, None] * c[None, :] * 4 * np.pi / p ** 3))
+    q1 = np.array([8.0])
+    c1 = np.ascontiguousarray(C[:1])
+    n = bas.n
+    S1, V1, K1 = (np.zeros((n, n), order="F") for _ in range(3))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    lib.zvo_one_e(1, P(q1), P(c1), P(S1), P(V1), P(K1))
+    assert math.isclose(V1[i, i], own, rel_tol=1e-13)
