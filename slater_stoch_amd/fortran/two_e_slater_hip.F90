@@ -11,6 +11,11 @@
 !                     executable)
 !   ZVMC_PRECISION    fp64_parity | fp64 (default) | fp32   (estimator)
 !   ZVMC_SVD_CLEAN    0 to skip the cleanup (default: run it, as the reference)
+!   ZVMC_ONE_E        1 to also write the one-electron integrals over the
+!                     fits as the reference's one_e_slater does
+!                     (overlap_ao_ex, nuclear_ao_ex, kinetic_ao_ex;
+!                     src/one_e_slater.F90:44-46,123-136), with the deck's
+!                     ng0 (one_e_slater always uses 20)
 !   ZVMC_EXACT_MONOCENTER  1 for the reference's do_exact_monocenter branch
 !                     (src/two_e_slater.F90:74,318-365: exact one-centre
 !                     integrals, standard error 0); default off, as there
@@ -47,6 +52,7 @@ program two_e_slater_hip
   integer(c_int64_t) :: nint, kcp, nscr
   integer(c_int), allocatable :: is(:), js(:), ks(:), ls(:)
   double precision, allocatable :: ijkl_gaus(:), precond(:), mean(:), err(:), exact(:)
+  double precision, allocatable :: s1e(:,:), v1e(:,:), k1e(:,:), qnuc(:)
   integer(c_int64_t) :: nmono
   double precision :: error_max, errmoy, dmoy, enucl, rij
   type(c_ptr) :: ctx
@@ -169,6 +175,29 @@ program two_e_slater_hip
   call zv_check(ctx, zv_set_verbose(ctx, 1_c_int), 'zv_set_verbose')
   call zv_check(ctx, zv_set_basis(ctx, n, center, npower, n_sto, i_type, zeta, n_gauss, ldg, &
       c_gauss, g_gauss, a_zv), 'zv_set_basis')
+  call get_environment_variable('ZVMC_ONE_E', env)
+  if (trim(env) == '1') then
+    allocate(s1e(n,n), v1e(n,n), k1e(n,n), qnuc(natom))
+    do i = 1, natom
+      qnuc(i) = dble(znum(atom(i)))
+    end do
+    call zv_check(ctx, zv_one_e(ctx, int(natom,c_int), qnuc, xyz, s1e, v1e, k1e), 'zv_one_e')
+    open(unit=104, file='overlap_ao_ex')
+    open(unit=105, file='nuclear_ao_ex')
+    open(unit=106, file='kinetic_ao_ex')
+    do i = 1, n
+      do k = 1, n
+        write(104,'(2(I5,X),D22.15)') i, k, s1e(i,k)
+        write(105,'(2(I5,X),D22.15)') i, k, v1e(i,k)
+        write(106,'(2(I5,X),D22.15)') i, k, k1e(i,k)
+      end do
+    end do
+    close(104)
+    close(105)
+    close(106)
+    deallocate(s1e, v1e, k1e, qnuc)
+  end if
+
   nint = zv_nint(n)
   allocate(is(nint), js(nint), ks(nint), ls(nint))
   allocate(ijkl_gaus(nint), precond(n*n), mean(nint), err(nint))

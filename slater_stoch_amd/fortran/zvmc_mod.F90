@@ -69,6 +69,14 @@ module zvmc
       integer(c_int64_t), intent(out) :: n_mono
     end function
 
+    integer(c_int) function zv_one_e(ctx, nnuc, charge, nuc, s, v, k) bind(C, name="zv_one_e")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nnuc
+      real(c_double), intent(in) :: charge(*), nuc(*)
+      real(c_double), intent(out) :: s(*), v(*), k(*)
+    end function
+
     integer(c_int) function zv_gauss_ijkl_list(ctx, nq, qi, qk, qj, ql, out) &
         bind(C, name="zv_gauss_ijkl_list")
       import :: c_int, c_ptr, c_double, c_int64_t

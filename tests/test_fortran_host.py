@@ -96,3 +96,22 @@ def test_exact_monocenter_branch(tmp_path):
     assert nm == 122 and (err == 0).sum() >= 122
     np.testing.assert_allclose(val, want, rtol=1e-14, atol=1e-300)
     np.testing.assert_allclose(err, werr, rtol=1e-14, atol=1e-300)
+
+
+@pytest.mark.gpu
+def test_one_e_files(tmp_path):
+    """ZVMC_ONE_E=1 writes the reference's one_e_slater files
+    (src/one_e_slater.F90:123-136) with the C-ABI's S, V, K."""
+    bas, p = _run(tmp_path, npts=80, env={"ZVMC_SVD_CLEAN": "0", "ZVMC_ONE_E": "1"})
+    assert p.returncode == 0, p.stdout + p.stderr
+    from slater_stoch_amd.zvmc import Engine
+    with Engine(0) as eng:
+        eng.set_basis(bas)
+        want = eng.one_e()
+    for name, M in zip(("overlap_ao_ex", "nuclear_ao_ex", "kinetic_ao_ex"), want):
+        lines = (tmp_path / name).read_text().splitlines()
+        assert len(lines) == bas.n ** 2 and all(len(ln) == 2 * 6 + 22 for ln in lines)
+        ik = np.array([[int(ln[0:5]), int(ln[6:11])] for ln in lines])
+        val = np.array([float(ln[12:34].replace("D", "E")) for ln in lines])
+        assert np.array_equal(ik[:, 0], np.repeat(np.arange(1, bas.n + 1), bas.n))
+        np.testing.assert_allclose(val, M[ik[:, 0] - 1, ik[:, 1] - 1], rtol=1e-14, atol=1e-300)
