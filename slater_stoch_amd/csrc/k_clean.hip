@@ -234,6 +234,16 @@ __global__ void k_pivot_check(const double *G, int r, double thr, rocblas_int *f
     if (threadIdx.x == 0) *flag = (mn[0] > thr * mx[0]) ? 0 : 1;
 }
 
+// G[t] = sum_b part[b * rr + t], b < nb
+__global__ void k_sum_parts(int64_t rr, int nb, const double *part, double *G)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rr) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[(int64_t)b * rr + t];
+    G[t] = s;
+}
+
 } // namespace
 
 // one rocBLAS handle per context, bound to the context's stream
@@ -289,7 +299,43 @@ static int qr_inplace(zv_context *c, rocblas_handle h, double *Z, int64_t N, int
 struct OrthoWork {
     double *G, *backup, *tau;
     rocblas_int *info;
+    double *part; // kGramSplit + 1 partial r x r products
 };
+
+// C (r x r) = A^T B for N x r panels A, B (leading dimension N): rocBLAS runs
+// the single r x r x N product at 3.3 TFLOP/s (N = 40000, r = 512: too few
+// output tiles), so the rows are split into kGramSplit chunks computed as one
+// strided batch (plus the remainder rows), then summed.
+constexpr int kGramSplit = 16;
+
+static int gram_tn(zv_context *c, rocblas_handle h, const double *A, const double *B, int64_t N,
+                   int r, double *C, double *part)
+{
+    const double one = 1.0, zero = 0.0;
+    const int64_t chunk = N / kGramSplit, rr = (int64_t)r * r;
+    if (chunk < 256) {
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r,
+                            (rocblas_int)N, &one, A, (rocblas_int)N, B, (rocblas_int)N, &zero, C,
+                            r));
+        return ZV_OK;
+    }
+    RB(c, rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, r,
+                                        r, (rocblas_int)chunk, &one, A, (rocblas_int)N, chunk, B,
+                                        (rocblas_int)N, chunk, &zero, part, r, rr, kGramSplit));
+    const int64_t rest = N - chunk * kGramSplit;
+    int nb = kGramSplit;
+    if (rest > 0) {
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r,
+                            (rocblas_int)rest, &one, A + chunk * kGramSplit, (rocblas_int)N,
+                            B + chunk * kGramSplit, (rocblas_int)N, &zero,
+                            part + (int64_t)kGramSplit * rr, r));
+        nb++;
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((rr + 255) / 256)), dim3(256), 0, c->stream,
+                       rr, nb, (const double *)part, C);
+    ZV_HIPCHECK(c, hipGetLastError());
+    return ZV_OK;
+}
 
 // Z (N x r, N >= r) <- an orthonormal basis of its columns, in place:
 // shifted CholeskyQR3; Householder QR from the saved panel when one of the
@@ -307,9 +353,7 @@ static int orthonormalize(zv_context *c, rocblas_handle h, double *Z, int64_t N,
     ZV_HIPCHECK(c, hipMemcpyAsync(w.backup, Z, sizeof(double) * N * r, hipMemcpyDeviceToDevice,
                                   c->stream));
     for (int round = 0; round < 3; round++) {
-        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r,
-                            (rocblas_int)N, &one, Z, (rocblas_int)N, Z, (rocblas_int)N, &zero,
-                            w.G, r));
+        ZV_CHECK(gram_tn(c, h, Z, Z, N, r, w.G, w.part));
         if (round == 0) {
             hipLaunchKernelGGL(k_shift_diag, dim3(1), dim3(256), 0, c->stream, w.G, r, f);
             ZV_HIPCHECK(c, hipGetLastError());
@@ -355,7 +399,9 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
     rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
     // power iterations keep their panel copy in S, the SVD's in U (both are
     // written only after the SVD)
-    const OrthoWork ow_iter{G, S, tau, info + 1}, ow_svd{G, U, tau, info + 1};
+    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * (kGramSplit + 1) * rr));
+    double *part = (double *)c->b_part.p;
+    const OrthoWork ow_iter{G, S, tau, info + 1, part}, ow_svd{G, U, tau, info + 1, part};
     int n_fallback = 0;
     ZV_CHECK(build_W(c, eri, d_eri, Ww));
     ZV_HIPCHECK(c, hipMemcpyAsync(W, Ww, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
@@ -396,8 +442,7 @@ int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
         ZV_HIPCHECK(c, hipMemcpyAsync(S, Yt, sizeof(double) * nr, hipMemcpyDeviceToDevice,
                                       c->stream));
         ZV_CHECK(orthonormalize(c, h, S, N, r, ow_svd, &n_fallback));
-        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, r, N, &one,
-                            Yt, N, S, N, &zero, Rt, r));
+        ZV_CHECK(gram_tn(c, h, Yt, S, N, r, Rt, part));
         // one-sided Jacobi: accurate singular vectors inside the clusters of
         // the late passes (rocSOLVER's divide-and-conquer gesdd lost the
         // small negative eigenvalues of a 110-AO ERI matrix there; gesvd is
