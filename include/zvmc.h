@@ -17,8 +17,12 @@
  *    src/build_mapping_ijkl.F90:16-43 (kcp = 1..nint).
  *  - Every call returns 0 on success or a ZV_ERR_* code; zv_last_error()
  *    returns the message.  The Fortran host turns nonzero into `stop`.
- *  - One HIP stream per context; calls are synchronous for the caller.
- *    Contexts are not thread-safe: call from one host thread.
+ *  - One HIP stream per context (plus side streams joined back inside a
+ *    call); calls are synchronous for the caller.  A context is not
+ *    thread-safe: drive it from one host thread.  Calls on different
+ *    contexts from several threads run one at a time (a process-wide lock
+ *    in every call that creates, drives or destroys a context): three
+ *    threads driving three contexts at once hung on MI355X.
  */
 #ifndef ZVMC_H
 #define ZVMC_H

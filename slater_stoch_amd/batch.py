@@ -18,16 +18,15 @@ are this package's:
   complete nbl-block estimate, as two_e_slater writes it.  ``samples``
   counts every run.
 
-Molecules run largest first on ``workers`` host threads, each thread with
-its own Engine (one library context: its own HIP streams and device
-buffers; the library keeps no state shared between contexts and ctypes
-releases the interpreter lock during calls).  Results are deterministic:
-molecule t uses Philox seed ``seed + t`` and stream r for its run r,
-whatever the worker count.  The default is one worker: a context uses five
-streams, and with several contexts the streams outnumber the device's
-hardware queues (4 by default), so unrelated streams share a queue and wait
-behind each other's long estimator launches -- on MI355X four workers made a
-150-AO molecule's analytic phase 60x slower (169 s instead of 2.8 s).
+Molecules run largest first through one Engine (one library context).
+Results are deterministic: molecule t uses Philox seed ``seed + t`` and
+stream r for its run r.  Several host threads, each driving its own context,
+are not supported: on MI355X four such workers made a 150-AO molecule's
+analytic phase 60x slower (169 s instead of 2.8 s: a context uses five
+streams and HIP opens four hardware queues per process by default, so
+unrelated streams waited behind each other's long estimator launches), and
+a run with three workers hung.  The library runs calls on different
+contexts one at a time, and ``workers`` must be 1.
 
 Command line (the 64-molecule batch of uneven sizes, FP32 estimator):
 
@@ -37,9 +36,7 @@ import argparse
 import json
 import math
 import sys
-import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -92,42 +89,26 @@ def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=400
 def run_batch(bases, target=1e-6, precision="fp32", workers=1, device=0, seed=0,
               callback=None, verbose=0, **policy):
     """Run every basis of ``bases``; returns the per-molecule dicts in input
-    order.  ``callback(index, result)`` is called as each molecule finishes
-    (from a worker thread; calls are serialized).  ``verbose`` is the
-    engines' progress level (2: a line per finished Monte Carlo block).
-    ``policy`` goes to run_molecule (nbl, pilot_npts, max_samples, safety,
-    max_rounds, keep_arrays)."""
+    order.  ``callback(index, result)`` is called as each molecule finishes.
+    ``verbose`` is the engine's progress level (2: a line per finished Monte
+    Carlo block).  ``policy`` goes to run_molecule (nbl, pilot_npts,
+    max_samples, safety, max_rounds, keep_arrays).  ``workers`` must be 1
+    (concurrent contexts are not supported, see the module docstring)."""
+    if int(workers) != 1:
+        raise ValueError("run_batch: workers must be 1 (contexts driven from several host "
+                         "threads at once are not supported)")
     from .zvmc import Engine
 
     order = sorted(range(len(bases)), key=lambda t: -bases[t].nint)
     results = [None] * len(bases)
-    local = threading.local()
-    engines, lock = [], threading.Lock()
-
-    def engine():
-        if getattr(local, "eng", None) is None:
-            local.eng = Engine(device, verbose=verbose)
-            with lock:
-                engines.append(local.eng)
-        return local.eng
-
-    def one(t):
-        r = run_molecule(engine(), bases[t], target=target, precision=precision,
-                         seed=seed + t, **policy)
-        r["index"] = t
-        results[t] = r
-        if callback is not None:
-            with lock:
+    with Engine(device, verbose=verbose) as eng:
+        for t in order:
+            r = run_molecule(eng, bases[t], target=target, precision=precision, seed=seed + t,
+                             **policy)
+            r["index"] = t
+            results[t] = r
+            if callback is not None:
                 callback(t, r)
-        return r
-
-    try:
-        with ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
-            for f in [pool.submit(one, t) for t in order]:
-                f.result()
-    finally:
-        for e in engines:
-            e.close()
     return results
 
 
@@ -150,7 +131,8 @@ def main(argv=None):
     ap.add_argument("--max-ao", type=int, default=150)
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--target", type=float, default=1e-6)
-    ap.add_argument("--workers", type=int, default=1)
+    ap.add_argument("--workers", type=int, default=1,
+                    help="must be 1: concurrent contexts are not supported")
     ap.add_argument("--max-samples", type=int, default=10_000_000)
     ap.add_argument("--max-rounds", type=int, default=4)
     ap.add_argument("--verbose", type=int, default=2,

@@ -4,8 +4,22 @@
 // k_clean.hip (PSD cleanup).
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 
 #include "zvmc_internal.h"
+
+// Every entry point that creates, drives or destroys a context holds one
+// process-wide lock for the whole call (calls are synchronous), so contexts
+// driven from several host threads run one call at a time: a run with three
+// host threads, each driving its own context at the same time, hung on
+// MI355X, and concurrent contexts are not supported.  Recursive: zv_create
+// calls zv_destroy on failure.
+static std::recursive_mutex &call_lock()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+#define ZV_SERIALIZE std::lock_guard<std::recursive_mutex> zv_serial_(call_lock())
 
 int zv_fail(zv_context *c, int code, const char *fmt, ...)
 {
@@ -164,6 +178,7 @@ extern "C" int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls)
 // ---------------------------------------------------------------- context
 extern "C" int zv_create(int device, zv_context **out)
 {
+    ZV_SERIALIZE;
     if (!out) return ZV_ERR_ARG;
     *out = nullptr;
     int ndev = 0;
@@ -207,6 +222,7 @@ static void free_basis(zv_context *c)
 
 extern "C" int zv_destroy(zv_context *c)
 {
+    ZV_SERIALIZE;
     if (!c) return ZV_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -284,6 +300,7 @@ extern "C" int zv_set_basis(zv_context *c, int n, const double *center, const in
                             const int *n_gauss, int ldg, const double *c_gauss,
                             const double *g_gauss, double a_zv)
 {
+    ZV_SERIALIZE;
     if (!c) return ZV_ERR_ARG;
     if (n <= 0 || n > 1024) return zv_fail(c, ZV_ERR_ARG, "nbasis %d out of range 1..1024", n);
     if (!center || !npower || !n_sto || !i_type || !zeta || !n_gauss || !c_gauss || !g_gauss)
@@ -438,6 +455,7 @@ extern "C" int zv_set_basis(zv_context *c, int n, const double *center, const in
 
 extern "C" int zv_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_screened)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (!ijkl) return zv_fail(c, ZV_ERR_ARG, "null output");
     int rc = zv_run_gauss_eri(c, ijkl, precond, n_screened);
@@ -447,6 +465,7 @@ extern "C" int zv_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_
 
 extern "C" int zv_exact_monocenter(zv_context *c, double *ijkl, double *err, int64_t *n_mono)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (!ijkl) return zv_fail(c, ZV_ERR_ARG, "null ijkl");
     int rc = zv_run_exact_monocenter(c, ijkl, err, n_mono);
@@ -457,6 +476,7 @@ extern "C" int zv_exact_monocenter(zv_context *c, double *ijkl, double *err, int
 extern "C" int zv_one_e(zv_context *c, int nnuc, const double *charge, const double *nuc,
                         double *S, double *V, double *K)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (nnuc < 1 || !charge || !nuc) return zv_fail(c, ZV_ERR_ARG, "no nuclei");
     if (!S || !V || !K) return zv_fail(c, ZV_ERR_ARG, "null output");
@@ -468,6 +488,7 @@ extern "C" int zv_one_e(zv_context *c, int nnuc, const double *charge, const dou
 extern "C" int zv_gauss_ijkl_list(zv_context *c, int64_t nq, const int *qi, const int *qk,
                                   const int *qj, const int *ql, double *out)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (nq < 0 || (nq > 0 && (!qi || !qk || !qj || !ql || !out)))
         return zv_fail(c, ZV_ERR_ARG, "bad quartet list");
@@ -487,6 +508,7 @@ extern "C" int zv_mc_blocks(zv_context *c, int nbl, int64_t npts, int mode, uint
                             uint64_t stream, int precision, const double *r1, const double *r2,
                             const double *pi0, double *mean, double *err, double *block_means)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (nbl < 1) return zv_fail(c, ZV_ERR_ARG, "nbl must be >= 1");
     if (npts < ZV_NW || npts % ZV_NW != 0)
@@ -507,6 +529,7 @@ extern "C" int zv_mc_blocks(zv_context *c, int nbl, int64_t npts, int mode, uint
 extern "C" int zv_draw_samples(zv_context *c, uint64_t seed, uint64_t stream, int64_t m0,
                                int64_t m, double *r1, double *r2, double *pi0)
 {
+    ZV_SERIALIZE;
     if (!c) return ZV_ERR_ARG;
     ZV_HIPCHECK(c, hipSetDevice(c->device));
     zv_reset_timings(c);
@@ -523,6 +546,7 @@ extern "C" int zv_pair_samples(zv_context *c, int i, int k, int ns, const double
                                double *rho2, double *rhog1, double *rhog2, double *jac1,
                                double *jac2)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (i < 1 || i > c->n || k < 1 || k > i)
         return zv_fail(c, ZV_ERR_ARG, "pair (%d,%d) must satisfy 1 <= k <= i <= %d", i, k, c->n);
@@ -539,6 +563,7 @@ extern "C" int zv_pair_samples(zv_context *c, int i, int k, int ns, const double
 extern "C" int zv_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri,
                             int *n_removed)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (!eri) return zv_fail(c, ZV_ERR_ARG, "null eri");
     if (rank < 1 || q < 0) return zv_fail(c, ZV_ERR_ARG, "rank >= 1 and q >= 0 required");
@@ -549,6 +574,7 @@ extern "C" int zv_svd_clean(zv_context *c, int rank, int q, uint64_t seed, doubl
 
 extern "C" int zv_davidson_clean(zv_context *c, double *eri, int *n_removed)
 {
+    ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
     if (!eri) return zv_fail(c, ZV_ERR_ARG, "null eri");
     int rc = zv_run_davidson_clean(c, eri, n_removed);

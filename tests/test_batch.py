@@ -1,9 +1,11 @@
-"""The batch driver: sample-size policy and worker independence.
+"""The batch driver: sample-size policy and determinism.
 
 CPU tests drive run_molecule with a stand-in engine whose standard error
-falls as 1/sqrt(samples); the GPU test runs real molecules and checks that
-the batch gives the same bits as one-molecule runs with the same seeds,
-whatever the number of worker threads."""
+falls as 1/sqrt(samples), and check that run_batch refuses several worker
+threads; the GPU test runs real molecules and checks that the batch gives
+the same bits as one-molecule runs with the same seeds.  The case of three
+contexts driven from three threads at once is kept, skipped: it hung on
+MI355X, and concurrent contexts are disabled."""
 import numpy as np
 import pytest
 
@@ -75,21 +77,61 @@ def test_policy_skips_production_when_the_pilot_suffices():
     assert r["rounds"] == 1 and r["reached"] and r["samples"] == 40000
 
 
-@pytest.mark.gpu
-def test_batch_matches_single_runs_for_any_worker_count():
-    from slater_stoch_amd.zvmc import Engine
+def test_batch_rejects_several_workers():
+    with pytest.raises(ValueError):
+        B.run_batch([FakeBasis()], workers=3)
 
+
+def _batch_case():
     bases = [b for b in configs.batch() if b.n <= 30][:4]
     assert len(bases) >= 3
     pol = dict(nbl=4, pilot_npts=400, max_samples=200_000, max_rounds=3, keep_arrays=True)
-    one = B.run_batch(bases, target=1e-7, precision="fp64", workers=1, seed=11, **pol)
-    three = B.run_batch(bases, target=1e-7, precision="fp64", workers=3, seed=11, **pol)
+    return bases, pol
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_runs():
+    from slater_stoch_amd.zvmc import Engine
+
+    bases, pol = _batch_case()
+    one = B.run_batch(bases, target=1e-7, precision="fp64", seed=11, **pol)
     with Engine(0) as e:
         for t, b in enumerate(bases):
             ref = B.run_molecule(e, b, target=1e-7, precision="fp64", seed=11 + t, **pol)
-            for got in (one[t], three[t]):
-                assert got["rounds"] == ref["rounds"] and got["samples"] == ref["samples"]
-                np.testing.assert_array_equal(got["eri"], ref["eri"])
-                np.testing.assert_array_equal(got["stderr"], ref["stderr"])
+            got = one[t]
+            assert got["rounds"] == ref["rounds"] and got["samples"] == ref["samples"]
+            np.testing.assert_array_equal(got["eri"], ref["eri"])
+            np.testing.assert_array_equal(got["stderr"], ref["stderr"])
     # the tight target forces production runs on at least one molecule
     assert max(r["rounds"] for r in one) >= 2
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="three host threads driving three contexts at once hung on MI355X "
+                         "(twice, once with the estimator's side streams switched off); "
+                         "concurrent contexts are disabled: run_batch rejects workers > 1 "
+                         "and the library serializes calls across contexts")
+def test_three_contexts_driven_at_once_match_single_runs():
+    import threading
+
+    from slater_stoch_amd.zvmc import Engine
+
+    bases, pol = _batch_case()
+    out = [None] * len(bases)
+
+    def work(ts):
+        with Engine(0) as e:
+            for t in ts:
+                out[t] = B.run_molecule(e, bases[t], target=1e-7, precision="fp64",
+                                        seed=11 + t, **pol)
+
+    threads = [threading.Thread(target=work, args=(list(range(k, len(bases), 3)),))
+               for k in range(3)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    with Engine(0) as e:
+        for t, b in enumerate(bases):
+            ref = B.run_molecule(e, b, target=1e-7, precision="fp64", seed=11 + t, **pol)
+            np.testing.assert_array_equal(out[t]["eri"], ref["eri"])
