@@ -117,22 +117,38 @@ def _traffic(name):
 
 
 def roofline_eri(res):
-    """The heaviest ERI kernel of the last step: algorithmic FP64 flops
-    (Boys values and R table once per primitive quartet, Hermite contractions
-    per component, as counted by the library) over its measured kernel time
-    (HIP events on the library's stream)."""
+    """The analytic ERI kernels of the last step: the shell-kernel buckets of
+    one Schwarz phase run side by side on the library's five streams and are
+    timed together (HIP events around the fork and join), like the nine
+    estimator kernels of a block.  Algorithmic FP64 flops (Boys values and R
+    table once per primitive quartet, Hermite contractions per component, as
+    counted by the library) over the phase time; a "launch" is one phase's
+    set of buckets."""
     st = res["eri_stats"]
-    key, d = max(st.items(), key=lambda kv: kv[1]["ms"])
-    name = kernel_name(key)
-    achieved = d["flops"] / (d["ms"] / 1e3) / 1e12
-    total_flops = sum(v["flops"] for v in st.values())
-    total_ms = sum(v["ms"] for v in st.values())
-    return {"bound": "fp64-valu", "kernel": name, "achieved": achieved,
-            "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-            "traffic": _traffic(name), "launches": d["launches"],
-            "total_ms": d["ms"], "avg_launch_ms": d["ms"] / max(1, d["launches"]),
-            "flops_per_launch": d["flops"] / max(1, d["launches"]),
-            "all_eri_kernels_tflops": total_flops / (total_ms / 1e3) / 1e12}
+    flops = sum(v["flops"] for v in st.values())
+    ms, nset = res["an_timings"]["analytic_eri"]
+    achieved = flops / (ms / 1e3) / 1e12
+    heavy = max(st.items(), key=lambda kv: kv[1]["flops"])
+    # HBM bytes: the committed per-kernel, per-launch traffic summed over the
+    # launches of the step (None unless every kernel is in the table)
+    traffic = 0.0
+    for key, d in st.items():
+        t = _traffic(kernel_name(key))
+        if t is None or traffic is None:
+            traffic = None
+            continue
+        traffic += t * d["launches"]
+    return {"bound": "fp64-valu",
+            "kernel": "k_eri_shell<LU, LL, B> / k_eri_shell_red<LU, LL, B> (the buckets of a "
+                      "Schwarz phase, side by side on five streams)",
+            "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64,
+            "traffic": traffic / max(1, nset) if traffic is not None else None,
+            "launches": nset, "kernel_launches": int(sum(v["launches"] for v in st.values())),
+            "total_ms": ms, "avg_launch_ms": ms / max(1, nset),
+            "flops_per_launch": flops / max(1, nset),
+            "heaviest_kernel": kernel_name(heavy[0]),
+            "heaviest_kernel_flop_share": heavy[1]["flops"] / flops}
 
 
 def shell_firsts(bas):
@@ -246,26 +262,25 @@ def density_report(res, bas, prec):
 
 def eri_by_class(res):
     """Per (LU, LL) shell class of the analytic phase (both lane batches and
-    modes together): kernel ms, primitive quartet x lane component
-    evaluations per second, algorithmic FP64 TFLOP/s and its fraction of the
-    78.6 TFLOP/s peak; heaviest first."""
+    modes together): algorithmic FP64 flops, their share of the phase, and
+    primitive quartet x lane component evaluations; heaviest first.  The
+    classes run side by side, so the phase time is not split among them."""
     agg = {}
     for key, d in res["eri_stats"].items():
         k = (key[1], key[2]) if key[0] == "shell" else ("component", key[1], key[2])
-        a = agg.setdefault(k, {"ms": 0.0, "units": 0.0, "flops": 0.0})
-        for f in ("ms", "units", "flops"):
+        a = agg.setdefault(k, {"units": 0.0, "flops": 0.0})
+        for f in ("units", "flops"):
             a[f] += d[f]
-    out = []
-    for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["ms"]):
-        sec = max(a["ms"], 1e-9) / 1e3
-        out.append({"class": list(k), "ms": a["ms"], "prim_quartet_components_per_s":
-                    a["units"] / sec, "tflops": a["flops"] / sec / 1e12,
-                    "frac": a["flops"] / sec / 1e12 / PEAK_FP64})
-    return out
+    total = sum(a["flops"] for a in agg.values()) or 1.0
+    return [{"class": list(k), "flops": a["flops"], "flop_share": a["flops"] / total,
+             "prim_quartet_components": a["units"]}
+            for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["flops"])]
 
 
 def roofline_of(res, bas):
-    """The dominant kernel of the step (largest total kernel time)."""
+    """The dominant launch of the step (largest total time): the estimator's
+    class-pair kernels of a block or the ERI buckets of a Schwarz phase, each
+    set timed together."""
     eri, est = roofline_eri(res), roofline_estimator(res, bas, "fp64")
     return (est, eri) if est["total_ms"] >= eri["total_ms"] else (eri, est)
 

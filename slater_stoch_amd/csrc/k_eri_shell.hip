@@ -731,19 +731,46 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
                                   c->stream));
     const SPD sd = spd_of(c);
     const ShellItem *di = (const ShellItem *)bitem.p;
+    // The buckets write disjoint partial sums, so they run side by side on
+    // the context's stream and its side streams (a bucket's last items leave
+    // CUs idle that the next bucket fills), joined back before the scatter.
+    // Largest planned cost first, each to the stream with the least planned
+    // cost so far.
     // bucket b = 25 v + 5 LU + LL, v = 2 (clb - 1) + red: statistics slot 36 + b
-    for (int b = 0; b < 100; b++) {
+    constexpr int NST = 1 + zv_context::NSIDE;
+    hipStream_t sts[NST] = {c->stream};
+    for (int k = 0; k < zv_context::NSIDE; k++) sts[1 + k] = c->side[k];
+    int order[100], nb = 0;
+    for (int b = 0; b < 100; b++)
+        if (pl.lo[b + 1] > pl.lo[b]) order[nb++] = b;
+    std::sort(order, order + nb, [&](int x, int y) { return pl.flops[x] > pl.flops[y]; });
+    double load[NST] = {};
+    // phase time: one span around the fork and join; the buckets' own spans
+    // overlap and only go to their statistics slots
+    PhaseTimer wall(c, PH_ERI);
+    ZV_HIPCHECK(c, hipEventRecord(c->ev_fork, c->stream));
+    for (int k = 0; k < zv_context::NSIDE; k++)
+        ZV_HIPCHECK(c, hipStreamWaitEvent(c->side[k], c->ev_fork, 0));
+    for (int t = 0; t < nb; t++) {
+        const int b = order[t];
         const int n = pl.lo[b + 1] - pl.lo[b];
-        if (!n) continue;
         const int v = b / 25, k = b % 25, LU = k / 5, LL = k % 5;
         const int cls = 36 + b;
         c->cls_pq[cls] += pl.units[b];
         c->cls_flops[cls] += pl.flops[b];
-        PhaseTimer tm(c, PH_ERI, cls);
-        if (!launch_shell(LU, LL, v / 2 + 1, (v & 1) != 0, (unsigned)n, c->stream, pp, sd,
+        int s = 0;
+        for (int q = 1; q < NST; q++)
+            if (load[q] < load[s]) s = q;
+        load[s] += pl.flops[b];
+        PhaseTimer tm(c, PH_N, cls, sts[s]);
+        if (!launch_shell(LU, LL, v / 2 + 1, (v & 1) != 0, (unsigned)n, sts[s], pp, sd,
                           di + pl.lo[b], (double *)bpart.p))
             return zv_fail(c, ZV_ERR_STATE, "no shell kernel for (%d,%d)", LU, LL);
         ZV_HIPCHECK(c, hipGetLastError());
+    }
+    for (int k = 0; k < zv_context::NSIDE; k++) {
+        ZV_HIPCHECK(c, hipEventRecord(c->ev_join[k], c->side[k]));
+        ZV_HIPCHECK(c, hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
     }
     return ZV_OK;
 }

@@ -59,7 +59,8 @@ int zv_ensure(zv_context *c, DevBuf &b, size_t bytes)
 }
 
 // ---------------------------------------------------------------- timing
-PhaseTimer::PhaseTimer(zv_context *c_, Phase p, int cls_) : c(c_), ph(p), cls(cls_)
+PhaseTimer::PhaseTimer(zv_context *c_, Phase p, int cls_, hipStream_t st_)
+    : c(c_), ph(p), cls(cls_), st(st_ ? st_ : c_->stream)
 {
     auto take = [&]() -> hipEvent_t {
         if (c->ev_used == c->ev_pool.size()) {
@@ -72,13 +73,13 @@ PhaseTimer::PhaseTimer(zv_context *c_, Phase p, int cls_) : c(c_), ph(p), cls(cl
     a = take();
     b = take();
     // timing is best effort: a failed record only drops this measurement
-    if (a && hipEventRecord(a, c->stream) != hipSuccess) a = nullptr;
+    if (a && hipEventRecord(a, st) != hipSuccess) a = nullptr;
 }
 
 PhaseTimer::~PhaseTimer()
 {
     if (!a || !b) return;
-    if (hipEventRecord(b, c->stream) != hipSuccess) return;
+    if (hipEventRecord(b, st) != hipSuccess) return;
     c->pend_phase.push_back(ph);
     c->pend_cls.push_back(cls);
     c->pend_ev.emplace_back(a, b);
@@ -108,8 +109,10 @@ int zv_collect_timings(zv_context *c)
     for (size_t t = 0; t < c->pend_ev.size(); t++) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->pend_ev[t].first, c->pend_ev[t].second) == hipSuccess) {
-            c->t_ms[c->pend_phase[t]] += ms;
-            c->t_n[c->pend_phase[t]] += 1;
+            if (c->pend_phase[t] < PH_N) {
+                c->t_ms[c->pend_phase[t]] += ms;
+                c->t_n[c->pend_phase[t]] += 1;
+            }
             int k = c->pend_cls[t];
             if (k >= 0 && k < ZV_NSTAT) {
                 c->cls_ms[k] += ms;

@@ -25,6 +25,8 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// PH_N as a PhaseTimer's phase: a kernel-class span only (launches that
+// overlap other timed launches; the phase takes one span around them)
 enum Phase { PH_SAMPLE = 0, PH_DENS, PH_EST, PH_STATS, PH_PRIM, PH_ERI, PH_COPY, PH_CLEAN, PH_N };
 
 struct zv_context {
@@ -151,7 +153,8 @@ struct PhaseTimer {
     Phase ph;
     int cls;
     hipEvent_t a = nullptr, b = nullptr;
-    PhaseTimer(zv_context *c_, Phase p, int cls_ = -1);
+    hipStream_t st = nullptr; // the context's stream unless given
+    PhaseTimer(zv_context *c_, Phase p, int cls_ = -1, hipStream_t st_ = nullptr);
     ~PhaseTimer();
 };
 void zv_reset_timings(zv_context *c);
