@@ -77,8 +77,9 @@ int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls);
 
 /* Analytic Gaussian-fit ERIs with the reference's Schwarz driver,
  * src/two_e_slater.F90:219-284: precond(i,k) = (ik|ik) (out, n x n, may be
- * NULL), ijkl_gaus(kcp) for all nint quartets (out), n_screened = quartets
- * dropped by the 1e-15 Cauchy-Schwarz test (out, may be NULL).
+ * NULL), ijkl_gaus(kcp) for all nint quartets (out; NULL keeps it on the
+ * device only, see zv_get_result), n_screened = quartets dropped by the
+ * 1e-15 Cauchy-Schwarz test (out, may be NULL).
  * Per quartet this is gauss_ijkl (src/gauss_ijkl.F90:51-154). */
 int zv_gauss_eri(zv_context *ctx, double *ijkl_gaus, double *precond, int64_t *n_screened);
 
@@ -115,10 +116,27 @@ int zv_gauss_ijkl_list(zv_context *ctx, int64_t nq, const int *qi, const int *qk
  * (src/draw_configuration.F90:1-29) and ignores r1/r2/pi0.  Outputs:
  * mean(kcp) = block average of e_S - e_G, err(kcp) = the reference's
  * standard error sqrt|<x^2>-<x>^2|/sqrt(nbl), optional block_means(kcp,nbl)
- * (may be NULL). */
+ * (may be NULL).  A NULL mean or err keeps that array on the device only
+ * (zv_get_result). */
 int zv_mc_blocks(zv_context *ctx, int nbl, int64_t npts, int mode, uint64_t seed,
                  uint64_t stream, int precision, const double *r1, const double *r2,
                  const double *pi0, double *mean, double *err, double *block_means);
+
+/* Device-resident results.  No counterpart in the reference: for callers
+ * that keep the per-quartet arrays in HBM between calls (a sample-size
+ * policy that only needs the largest standard error, a benchmark that times
+ * the job without the PCIe copies).  The last zv_gauss_eri (ZV_RESULT_ERI)
+ * and zv_mc_blocks (ZV_RESULT_MEAN, ZV_RESULT_STDERR) results stay on the
+ * device in kcp order until the next such call or zv_set_basis; a NULL host
+ * output skips only the copy.  zv_get_result copies one to host (nint
+ * doubles); zv_result_max reduces it on the device.  ZV_ERR_STATE when the
+ * result has not been computed for the current basis.  Neither call changes
+ * what zv_last_timings reports. */
+#define ZV_RESULT_ERI 0
+#define ZV_RESULT_MEAN 1
+#define ZV_RESULT_STDERR 2
+int zv_get_result(zv_context *ctx, int which, double *host);
+int zv_result_max(zv_context *ctx, int which, double *value);
 
 /* Samples drawn by the device sampler for global sample indices
  * m0 .. m0+m-1 (ZV_SAMPLES_PHILOX), copied to the host: r1(3,m), r2(3,m),

@@ -579,7 +579,7 @@ int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_sc
     // the off-diagonal components of (x|x) get the test now that precond is known
     ZV_CHECK(zv_shell_scatter(c, c->b_sgroup1, c->b_part1, ng1, pre, tri));
     ZV_CHECK(run_generic_pairs(c, pp, gb, gk, tri));
-    ZV_CHECK(zv_run_scatter_ref(c, tri, ijkl));
+    ZV_CHECK(zv_keep_result(c, tri, ZV_RESULT_ERI, ijkl));
     if (precond) {
         int n = c->n;
         for (int i = 1; i <= n; i++)

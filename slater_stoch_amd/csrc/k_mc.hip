@@ -693,6 +693,62 @@ int zv_run_scatter_ref(zv_context *c, const double *d_tri, double *host_out)
     return ZV_OK;
 }
 
+int zv_keep_result(zv_context *c, const double *d_tri, int which, double *host_out)
+{
+    int64_t ntri = c->nint;
+    DevBuf &res = c->b_res[which];
+    c->res_valid[which] = false;
+    ZV_CHECK(zv_ensure(c, res, sizeof(double) * ntri));
+    {
+        PhaseTimer tm(c, PH_COPY);
+        hipLaunchKernelGGL(k_scatter_ref, dim3(nblk(ntri, 256)), dim3(256), 0, c->stream, ntri,
+                           d_tri, (double *)res.p, c->d_pair_i, c->d_pair_k, c->d_kbase, c->n);
+        ZV_HIPCHECK(c, hipGetLastError());
+        if (host_out)
+            ZV_HIPCHECK(c, hipMemcpyAsync(host_out, res.p, sizeof(double) * ntri,
+                                          hipMemcpyDeviceToHost, c->stream));
+    }
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->res_valid[which] = true;
+    return ZV_OK;
+}
+
+namespace {
+// per-workgroup maxima of x[0 .. n), grid-stride
+__global__ void k_block_max(int64_t n, const double *x, double *part)
+{
+    __shared__ double red[256];
+    double m = -INFINITY;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, x[t]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+} // namespace
+
+int zv_run_result_max(zv_context *c, int which, double *value)
+{
+    constexpr int nb = 1024;
+    ZV_CHECK(zv_ensure(c, c->b_out2, sizeof(double) * nb));
+    double *part = (double *)c->b_out2.p;
+    hipLaunchKernelGGL(k_block_max, dim3(nb), dim3(256), 0, c->stream, c->nint,
+                       (const double *)c->b_res[which].p, part);
+    ZV_HIPCHECK(c, hipGetLastError());
+    double h[nb];
+    ZV_HIPCHECK(c, hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    double m = -INFINITY;
+    for (int b = 0; b < nb; b++) m = fmax(m, h[b]);
+    *value = m;
+    return ZV_OK;
+}
+
 int zv_run_draw(zv_context *c, uint64_t seed, uint64_t stream, int64_t m0, int64_t m, double *r1,
                 double *r2, double *pi0)
 {
@@ -1083,7 +1139,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                            nbl, s1, s2, accS, (double *)c->b_out2.p);
         ZV_HIPCHECK(c, hipGetLastError());
     }
-    ZV_CHECK(zv_run_scatter_ref(c, accS, mean));
-    ZV_CHECK(zv_run_scatter_ref(c, (const double *)c->b_out2.p, err));
+    ZV_CHECK(zv_keep_result(c, accS, ZV_RESULT_MEAN, mean));
+    ZV_CHECK(zv_keep_result(c, (const double *)c->b_out2.p, ZV_RESULT_STDERR, err));
     return ZV_OK;
 }

@@ -221,6 +221,7 @@ static void free_basis(zv_context *c)
     c->pp_ready = false;
     c->sp_ready = false;
     c->grp_ready = false;
+    for (bool &v : c->res_valid) v = false;
 }
 
 extern "C" int zv_destroy(zv_context *c)
@@ -236,7 +237,7 @@ extern "C" int zv_destroy(zv_context *c)
                       &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc,
                       &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
                       &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist,
-                      &c->b_gmeta};
+                      &c->b_gmeta, &c->b_res[0], &c->b_res[1], &c->b_res[2]};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -460,7 +461,6 @@ extern "C" int zv_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_
 {
     ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
-    if (!ijkl) return zv_fail(c, ZV_ERR_ARG, "null output");
     int rc = zv_run_gauss_eri(c, ijkl, precond, n_screened);
     if (rc == ZV_OK) rc = zv_collect_timings(c);
     return rc;
@@ -522,11 +522,40 @@ extern "C" int zv_mc_blocks(zv_context *c, int nbl, int64_t npts, int mode, uint
         return zv_fail(c, ZV_ERR_ARG, "fixed-sample mode needs r1, r2, pi0");
     if (precision < ZV_PREC_FP64_PARITY || precision > ZV_PREC_FP32)
         return zv_fail(c, ZV_ERR_ARG, "unknown precision %d", precision);
-    if (!mean || !err) return zv_fail(c, ZV_ERR_ARG, "null output");
     int rc = zv_run_mc(c, nbl, npts, mode, seed, stream, precision, r1, r2, pi0, mean, err,
                        block_means);
     if (rc == ZV_OK) rc = zv_collect_timings(c);
     return rc;
+}
+
+extern "C" int zv_get_result(zv_context *c, int which, double *host)
+{
+    // a query: leaves the timings of the last computing call as they are
+    ZV_SERIALIZE;
+    if (!c) return ZV_ERR_ARG;
+    if (c->n <= 0) return zv_fail(c, ZV_ERR_STATE, "zv_set_basis has not been called");
+    ZV_HIPCHECK(c, hipSetDevice(c->device));
+    if (which < ZV_RESULT_ERI || which > ZV_RESULT_STDERR || !host)
+        return zv_fail(c, ZV_ERR_ARG, "bad result index %d or null output", which);
+    if (!c->res_valid[which])
+        return zv_fail(c, ZV_ERR_STATE, "result %d has not been computed for this basis", which);
+    ZV_HIPCHECK(c, hipMemcpy(host, c->b_res[which].p, sizeof(double) * c->nint,
+                             hipMemcpyDeviceToHost));
+    return ZV_OK;
+}
+
+extern "C" int zv_result_max(zv_context *c, int which, double *value)
+{
+    // a query: leaves the timings of the last computing call as they are
+    ZV_SERIALIZE;
+    if (!c) return ZV_ERR_ARG;
+    if (c->n <= 0) return zv_fail(c, ZV_ERR_STATE, "zv_set_basis has not been called");
+    ZV_HIPCHECK(c, hipSetDevice(c->device));
+    if (which < ZV_RESULT_ERI || which > ZV_RESULT_STDERR || !value)
+        return zv_fail(c, ZV_ERR_ARG, "bad result index %d or null output", which);
+    if (!c->res_valid[which])
+        return zv_fail(c, ZV_ERR_STATE, "result %d has not been computed for this basis", which);
+    return zv_run_result_max(c, which, value);
 }
 
 extern "C" int zv_draw_samples(zv_context *c, uint64_t seed, uint64_t stream, int64_t m0,

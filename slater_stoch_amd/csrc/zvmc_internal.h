@@ -77,6 +77,10 @@ struct zv_context {
     DevBuf b_s1, b_s2;                 // per quartet sums over blocks
     DevBuf b_out, b_out2;              // reference-order outputs
     DevBuf b_blk;                      // optional block means
+    // device-resident results in reference (kcp) order: ZV_RESULT_ERI,
+    // ZV_RESULT_MEAN, ZV_RESULT_STDERR of the last zv_gauss_eri / zv_mc_blocks
+    DevBuf b_res[3];
+    bool res_valid[3] = {false, false, false};
     DevBuf b_mpart;                    // estimator partial sums of sample splits 1..n-1
     DevBuf b_pp;                       // primitive-pair data
     DevBuf b_ppE;                      // primitive-pair Hermite coefficients
@@ -176,6 +180,10 @@ int zv_run_one_e(zv_context *c, int nnuc, const double *charge, const double *nu
 int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, const int *qj,
                       const int *ql, double *out);
 int zv_run_scatter_ref(zv_context *c, const double *d_tri, double *host_out);
+// d_tri (triangle order) -> result slot `which` in kcp order on the device,
+// then to host_out unless it is null
+int zv_keep_result(zv_context *c, const double *d_tri, int which, double *host_out);
+int zv_run_result_max(zv_context *c, int which, double *value);
 int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri, int *n_removed);
 int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed);
 

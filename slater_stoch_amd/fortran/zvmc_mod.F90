@@ -113,6 +113,18 @@ module zvmc
       real(c_double), intent(inout) :: eri(*)
       integer(c_int), intent(out) :: n_removed
     end function
+    integer(c_int) function zv_get_result(ctx, which, host) bind(C, name="zv_get_result")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: host(*)
+    end function
+    integer(c_int) function zv_result_max(ctx, which, val) bind(C, name="zv_result_max")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: val
+    end function
     integer(c_int) function zv_set_verbose(ctx, level) bind(C, name="zv_set_verbose")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

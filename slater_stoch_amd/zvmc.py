@@ -47,7 +47,11 @@ SIGNATURES = {
     "zv_set_verbose": (I32, [VP, I32]),
     "zv_last_timings": (I32, [VP, VP, VP]),
     "zv_last_eri_stats": (I32, [VP, VP, VP, VP, VP]),
+    "zv_get_result": (I32, [VP, I32, VP]),
+    "zv_result_max": (I32, [VP, I32, VP]),
 }
+
+RESULT_ERI, RESULT_MEAN, RESULT_STDERR = 0, 1, 2
 
 _LIB = None
 
@@ -138,10 +142,12 @@ class Engine:
             float(bas.a_zv)))
 
     # ------------------------------------------------------------ analytic
-    def gauss_eri(self):
-        """(ijkl_gaus[nint], precond[n, n] with precond[k-1, i-1] = (ik|ik), n_screened)."""
+    def gauss_eri(self, device_only=False):
+        """(ijkl_gaus[nint], precond[n, n] with precond[k-1, i-1] = (ik|ik), n_screened);
+        with device_only the ERIs stay on the device (get_result) and None is
+        returned in their place."""
         n = self.basis.n
-        out = np.zeros(self.basis.nint)
+        out = None if device_only else np.zeros(self.basis.nint)
         pre = np.zeros((n, n))
         nscr = np.zeros(1, np.int64)
         self._check(self.lib.zv_gauss_eri(self.h, _p(out), _p(pre), _p(nscr)))
@@ -178,6 +184,19 @@ class Engine:
         self._check(self.lib.zv_one_e(self.h, q.size, _p(q), _p(c), _p(S), _p(V), _p(K)))
         return S, V, K
 
+    def get_result(self, which):
+        """Device-resident result (RESULT_ERI, RESULT_MEAN, RESULT_STDERR) of
+        the last gauss_eri / mc_blocks, copied to a new host array."""
+        out = np.zeros(self.basis.nint)
+        self._check(self.lib.zv_get_result(self.h, int(which), _p(out)))
+        return out
+
+    def result_max(self, which):
+        """Largest entry of a device-resident result, reduced on the device."""
+        v = np.zeros(1)
+        self._check(self.lib.zv_result_max(self.h, int(which), _p(v)))
+        return float(v[0])
+
     def gauss_ijkl(self, qi, qk, qj, ql):
         qi, qk, qj, ql = [np.ascontiguousarray(a, np.int32) for a in (qi, qk, qj, ql)]
         out = np.zeros(qi.size)
@@ -187,12 +206,14 @@ class Engine:
 
     # ------------------------------------------------------------ MC
     def mc_blocks(self, nbl, npts, precision="fp64_parity", samples=None, seed=0, stream=0,
-                  block_means=False):
+                  block_means=False, device_only=False):
         """samples = (r1[M,3], r2[M,3], pi0[M]) for the fixed-sample mode
-        (M = nbl*npts), or None to draw on the device from (seed, stream)."""
+        (M = nbl*npts), or None to draw on the device from (seed, stream).
+        With device_only mean and err stay on the device (get_result,
+        result_max) and (None, None) is returned in their place."""
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         nint = self.basis.nint
-        mean, err = np.zeros(nint), np.zeros(nint)
+        mean, err = (None, None) if device_only else (np.zeros(nint), np.zeros(nint))
         blk = np.zeros((nbl, nint)) if block_means else None
         if samples is not None:
             r1, r2, pi0 = [np.ascontiguousarray(a, np.float64) for a in samples]

@@ -1,0 +1,67 @@
+"""GPU checks of the device-resident results (zv_get_result, zv_result_max).
+
+A NULL host output keeps the per-quartet array on the device; fetched
+afterwards it carries the same bits as the copy the call would have made,
+and the on-device maximum equals numpy's over that array.  A new basis
+invalidates the kept results."""
+import numpy as np
+import pytest
+
+from slater_stoch_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from slater_stoch_amd.zvmc import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def test_device_results_equal_host_outputs(engine):
+    from slater_stoch_amd import zvmc as Z
+    engine.set_basis(configs.water_minimal(ng0=20))
+    eri, pre, nscr = engine.gauss_eri()
+    none, pre2, nscr2 = engine.gauss_eri(device_only=True)
+    assert none is None and nscr2 == nscr
+    np.testing.assert_array_equal(pre2, pre)
+    kept = engine.get_result(Z.RESULT_ERI)
+    np.testing.assert_array_equal(kept, eri)
+    assert engine.result_max(Z.RESULT_ERI) == eri.max()
+
+    mean, err = engine.mc_blocks(10, 400, "fp64", seed=7, stream=3)
+    m2, e2 = engine.mc_blocks(10, 400, "fp64", seed=7, stream=3, device_only=True)
+    assert m2 is None and e2 is None
+    timings = engine.timings()
+    assert timings["estimator"][1] > 0
+    np.testing.assert_array_equal(engine.get_result(Z.RESULT_MEAN), mean)
+    np.testing.assert_array_equal(engine.get_result(Z.RESULT_STDERR), err)
+    assert engine.result_max(Z.RESULT_STDERR) == err.max()
+    assert engine.result_max(Z.RESULT_MEAN) == mean.max()
+    # the queries leave the timings of the last computing call alone
+    assert engine.timings() == timings
+
+
+def test_device_results_on_110_ao_molecule(engine):
+    """Sizes above one reduction pass of the maximum (18.6e6 quartets)."""
+    from slater_stoch_amd import zvmc as Z
+    engine.set_basis(configs.random_molecule())
+    engine.mc_blocks(10, 40, "fp32", seed=3, device_only=True)
+    err = engine.get_result(Z.RESULT_STDERR)
+    assert err.size == 18_638_565
+    assert engine.result_max(Z.RESULT_STDERR) == err.max()
+
+
+def test_new_basis_invalidates_results(engine):
+    from slater_stoch_amd import zvmc as Z
+    engine.set_basis(configs.water_minimal(ng0=4))
+    engine.gauss_eri(device_only=True)
+    engine.get_result(Z.RESULT_ERI)
+    engine.set_basis(configs.water_minimal(ng0=4))
+    for which in (Z.RESULT_ERI, Z.RESULT_MEAN, Z.RESULT_STDERR):
+        with pytest.raises(Z.ZVError):
+            engine.get_result(which)
+        with pytest.raises(Z.ZVError):
+            engine.result_max(which)
