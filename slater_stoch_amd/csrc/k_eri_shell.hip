@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "eri_device.h"
@@ -537,32 +538,42 @@ double item_cost(const zv_context *c, int su, int sl)
     return cost;
 }
 
-// Orientation, splitting and ordering of a list of shell quartets (x, y),
-// both shell pairs of order <= ZV_LMAX_T with kept primitive pairs.  Items
-// are cut along lane-component passes first (no extra work), then along
-// uniform primitive-pair slices (partial sums added by the scatter).
-void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq, Plan &pl)
+// Host threads for the planner: up to 8, one for short lists.
+int plan_threads(size_t n)
 {
-    struct Or {
-        int su, sl;
-        double cost;
-    };
-    std::vector<Or> ors;
-    ors.reserve(sq.size());
-    double total = 0.0;
-    for (const auto &xy : sq) {
-        const int x = xy.first, y = xy.second;
-        double cxy = item_cost(c, x, y), cyx = x == y ? cxy : item_cost(c, y, x);
-        Or o = cxy <= cyx ? Or{x, y, cxy} : Or{y, x, cyx};
-        ors.push_back(o);
-        total += o.cost;
-    }
-    // items of at most 1/40000 of the total cost: larger quartets are split,
-    // so the sorted queue ends on short items
-    const double budget = std::max(total / 40000.0, 2.0e6);
+    if (n < 20000) return 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(8u, hw ? hw : 1u));
+}
+
+// run f(t) for t = 0 .. nt-1, on nt - 1 extra threads and this one
+template <typename F> void run_threads(int nt, F &&f)
+{
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(f, t);
+    f(0);
+    for (auto &x : th) x.join();
+}
+
+// Items of one run of oriented shell quartets, as the sequential planner
+// would make them but with partial-sum offsets from 0.
+struct PlanPart {
+    std::vector<ShellGroup> groups;
     std::vector<ShellItem> tmp[100];
     std::vector<float> tcost[100];
-    for (const Or &o : ors) {
+    double units[100] = {}, flops[100] = {};
+    int64_t npart = 0;
+};
+
+struct Orient {
+    int su, sl;
+    double cost;
+};
+
+void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget, PlanPart &pp)
+{
+    for (size_t q = 0; q < n; q++) {
+        const Orient &o = ors[q];
         const int LU = c->h_sp_L[o.su], LL = c->h_sp_L[o.sl], L = LU + LL;
         const int ncu = c->h_sp_coff[o.su + 1] - c->h_sp_coff[o.su];
         const int ncl = c->h_sp_coff[o.sl + 1] - c->h_sp_coff[o.sl];
@@ -593,55 +604,125 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
                     const double cr = (p1 - p0) * wp;
                     const int ns =
                         (int)std::min<double>(npu, std::max(1.0, std::ceil(cr / budget)));
-                    ShellGroup g{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pl.npart};
-                    pl.groups.push_back(g);
+                    pp.groups.push_back(ShellGroup{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pp.npart});
                     // algorithmic flops: Boys and R once per primitive quartet;
                     // per lane component the X contraction, then either nu
                     // direct dot products per primitive quartet or the sum over
                     // lane primitives per primitive quartet plus nu dot
                     // products per uniform primitive
                     const double pq = (double)npu * npl;
-                    pl.units[b] += pq * nlr;
+                    pp.units[b] += pq * nlr;
                     double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
                     fl += red ? nlr * (pq * HU + (double)npu * 2.0 * nu * HU)
                               : nlr * pq * 2.0 * nu * HU;
-                    pl.flops[b] += fl;
+                    pp.flops[b] += fl;
                     for (int sl = 0; sl < ns; sl++) {
                         const int a0 = (int)((int64_t)npu * sl / ns);
                         const int a1 = (int)((int64_t)npu * (sl + 1) / ns);
-                        tmp[b].push_back(ShellItem{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
-                                                   pl.npart + (int64_t)sl * nu * nlr});
-                        tcost[b].push_back((float)(cr * (a1 - a0) / npu));
+                        pp.tmp[b].push_back(ShellItem{o.su, o.sl, a0, a1, l0, l1, cu0, nu,
+                                                      pp.npart + (int64_t)sl * nu * nlr});
+                        pp.tcost[b].push_back((float)(cr * (a1 - a0) / npu));
                     }
-                    pl.npart += (int64_t)ns * nu * nlr;
+                    pp.npart += (int64_t)ns * nu * nlr;
                 }
             }
         }
     }
-    // each bucket in descending cost (quarter-octave bins, counting sort), so the
-    // long items start first and the queue ends on short ones
-    size_t nitems = 0;
-    for (int b = 0; b < 100; b++) nitems += tmp[b].size();
-    pl.items.resize(nitems);
-    constexpr int NBIN = 256;
-    std::vector<uint8_t> bin;
+}
+
+// Orientation, splitting and ordering of a list of shell quartets (x, y),
+// both shell pairs of order <= ZV_LMAX_T with kept primitive pairs.  Items
+// are cut along lane-component passes first (no extra work), then along
+// uniform primitive-pair slices (partial sums added by the scatter).  Long
+// lists are planned on host threads over contiguous runs of quartets whose
+// items are joined in order, so the plan is the same for any thread count.
+void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq, Plan &pl)
+{
+    const size_t nq = sq.size();
+    const int nt = plan_threads(nq);
+    std::vector<Orient> ors(nq);
+    run_threads(nt, [&](int t) {
+        for (size_t q = nq * t / nt; q < nq * (t + 1) / nt; q++) {
+            const int x = sq[q].first, y = sq[q].second;
+            double cxy = item_cost(c, x, y), cyx = x == y ? cxy : item_cost(c, y, x);
+            ors[q] = cxy <= cyx ? Orient{x, y, cxy} : Orient{y, x, cyx};
+        }
+    });
+    double total = 0.0;
+    for (const Orient &o : ors) total += o.cost;
+    // items of at most 1/40000 of the total cost: larger quartets are split,
+    // so the sorted queue ends on short items
+    const double budget = std::max(total / 40000.0, 2.0e6);
+    std::vector<PlanPart> parts(nt);
+    run_threads(nt, [&](int t) {
+        const size_t q0 = nq * t / nt, q1 = nq * (t + 1) / nt;
+        plan_items(c, ors.data() + q0, q1 - q0, budget, parts[t]);
+    });
+    // join the runs in order: partial-sum offsets shifted by the runs before
+    pl.npart = 0;
+    size_t ng = 0, nb_items[100] = {};
+    for (const PlanPart &pp : parts) {
+        ng += pp.groups.size();
+        for (int b = 0; b < 100; b++) nb_items[b] += pp.tmp[b].size();
+    }
+    pl.groups.reserve(ng);
     size_t at = 0;
     for (int b = 0; b < 100; b++) {
         pl.lo[b] = (int)at;
-        const size_t n = tmp[b].size();
-        bin.resize(n);
-        int cnt[NBIN + 1] = {};
-        static_assert(NBIN <= 256, "bins are stored as bytes");
-        for (size_t t = 0; t < n; t++) {
-            const int e = (int)std::floor(4.0 * std::log2(std::max(tcost[b][t], 1.0f)));
-            bin[t] = (uint8_t)(NBIN - 1 - std::min(e, NBIN - 1)); // large cost -> small bin
-            cnt[bin[t] + 1]++;
-        }
-        for (int k = 0; k < NBIN; k++) cnt[k + 1] += cnt[k];
-        for (size_t t = 0; t < n; t++) pl.items[at + cnt[bin[t]]++] = tmp[b][t];
-        at += n;
+        at += nb_items[b];
     }
     pl.lo[100] = (int)at;
+    pl.items.resize(at);
+    std::vector<int64_t> off(nt, 0);
+    for (int t = 0; t < nt; t++) {
+        PlanPart &pp = parts[t];
+        off[t] = pl.npart;
+        for (ShellGroup g : pp.groups) {
+            g.part += off[t];
+            pl.groups.push_back(g);
+        }
+        for (int b = 0; b < 100; b++) {
+            pl.units[b] += pp.units[b];
+            pl.flops[b] += pp.flops[b];
+        }
+        pl.npart += pp.npart;
+    }
+    // each bucket in descending cost (quarter-octave bins, counting sort), so the
+    // long items start first and the queue ends on short ones; buckets are
+    // independent and spread over the threads
+    constexpr int NBIN = 256;
+    static_assert(NBIN <= 256, "bins are stored as bytes");
+    run_threads(nt, [&](int t) {
+        std::vector<uint8_t> bin;
+        std::vector<const ShellItem *> src;
+        std::vector<float> cst;
+        for (int b = t; b < 100; b += nt) {
+            const size_t n = nb_items[b];
+            if (!n) continue;
+            src.clear();
+            cst.clear();
+            std::vector<int64_t> shift;
+            for (int u = 0; u < nt; u++)
+                for (size_t i = 0; i < parts[u].tmp[b].size(); i++) {
+                    src.push_back(&parts[u].tmp[b][i]);
+                    cst.push_back(parts[u].tcost[b][i]);
+                    shift.push_back(off[u]);
+                }
+            bin.resize(n);
+            int cnt[NBIN + 1] = {};
+            for (size_t i = 0; i < n; i++) {
+                const int e = (int)std::floor(4.0 * std::log2(std::max(cst[i], 1.0f)));
+                bin[i] = (uint8_t)(NBIN - 1 - std::min(e, NBIN - 1)); // large cost -> small bin
+                cnt[bin[i] + 1]++;
+            }
+            for (int k = 0; k < NBIN; k++) cnt[k + 1] += cnt[k];
+            for (size_t i = 0; i < n; i++) {
+                ShellItem it = *src[i];
+                it.part += shift[i];
+                pl.items[pl.lo[b] + cnt[bin[i]]++] = it;
+            }
+        }
+    });
 }
 
 } // namespace
