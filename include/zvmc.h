@@ -80,7 +80,9 @@ int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls);
  * NULL), ijkl_gaus(kcp) for all nint quartets (out; NULL keeps it on the
  * device only, see zv_get_result), n_screened = quartets dropped by the
  * 1e-15 Cauchy-Schwarz test (out, may be NULL).
- * Per quartet this is gauss_ijkl (src/gauss_ijkl.F90:51-154). */
+ * Per quartet this is gauss_ijkl (src/gauss_ijkl.F90:51-154).  The work
+ * plan of long quartet lists is built on up to 8 host threads inside the
+ * call (joined before it returns). */
 int zv_gauss_eri(zv_context *ctx, double *ijkl_gaus, double *precond, int64_t *n_screened);
 
 /* Exact Slater integrals of the one-centre quartets (all four AOs on one
