@@ -541,33 +541,53 @@ __global__ __launch_bounds__(256) void k_est_group(
                             dz = u1[i][2] - u2[j][2];
                     K[i][j] = rsq(fma(dz, dz, fma(dy, dy, dx * dx)));
                 }
-            T ka[ER][TC], kb[ER][TC], c2[EC], d2[EC];
+            // K scales the side with fewer products: the row components
+            // (ER x TC per field) or the column components (TR x EC)
+            constexpr bool SCALE_COLS = TR * EC < ER * TC;
+            if constexpr (!SCALE_COLS) {
+                T ka[ER][TC], kb[ER][TC];
 #pragma unroll
-            for (int i = 0; i < ER; i++)
+                for (int i = 0; i < ER; i++)
 #pragma unroll
-                for (int j = 0; j < TC; j++) {
-                    ka[i][j] = K[i / NR][j] * a1[i];
-                    kb[i][j] = K[i / NR][j] * b1[i];
-                }
-#pragma unroll
-            for (int j = 0; j < EC; j++) {
-                c2[j] = a2[j];
-                d2[j] = b2[j];
-            }
-#pragma unroll
-            for (int i = 0; i < ER; i++)
-#pragma unroll
-                for (int j = 0; j < TC; j++)
-#pragma unroll
-                    for (int c = 0; c < NC; c++) {
-                        if constexpr (sizeof(T) == 8)
-                            acc64[i][j * NC + c] =
-                                fma(ka[i][j], c2[j * NC + c],
-                                    fma(kb[i][j], d2[j * NC + c], acc64[i][j * NC + c]));
-                        else
-                            a[i][j * NC + c] = fma(ka[i][j], c2[j * NC + c],
-                                                   fma(kb[i][j], d2[j * NC + c], a[i][j * NC + c]));
+                    for (int j = 0; j < TC; j++) {
+                        ka[i][j] = K[i / NR][j] * a1[i];
+                        kb[i][j] = K[i / NR][j] * b1[i];
                     }
+#pragma unroll
+                for (int i = 0; i < ER; i++)
+#pragma unroll
+                    for (int j = 0; j < TC; j++)
+#pragma unroll
+                        for (int c = 0; c < NC; c++) {
+                            if constexpr (sizeof(T) == 8)
+                                acc64[i][j * NC + c] =
+                                    fma(ka[i][j], a2[j * NC + c],
+                                        fma(kb[i][j], b2[j * NC + c], acc64[i][j * NC + c]));
+                            else
+                                a[i][j * NC + c] =
+                                    fma(ka[i][j], a2[j * NC + c],
+                                        fma(kb[i][j], b2[j * NC + c], a[i][j * NC + c]));
+                        }
+            } else {
+                T kc[TR][EC], kd[TR][EC];
+#pragma unroll
+                for (int u = 0; u < TR; u++)
+#pragma unroll
+                    for (int j = 0; j < EC; j++) {
+                        kc[u][j] = K[u][j / NC] * a2[j];
+                        kd[u][j] = K[u][j / NC] * b2[j];
+                    }
+#pragma unroll
+                for (int i = 0; i < ER; i++)
+#pragma unroll
+                    for (int j = 0; j < EC; j++) {
+                        if constexpr (sizeof(T) == 8)
+                            acc64[i][j] = fma(a1[i], kc[i / NR][j],
+                                              fma(b1[i], kd[i / NR][j], acc64[i][j]));
+                        else
+                            a[i][j] = fma(a1[i], kc[i / NR][j], fma(b1[i], kd[i / NR][j], a[i][j]));
+                    }
+            }
         }
         if constexpr (sizeof(T) != 8) {
 #pragma unroll
