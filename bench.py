@@ -84,7 +84,7 @@ def job(eng, bas, prec, seed):
 
 
 def precision_report(bas, m64, e64, m32):
-    """FP32-estimator error against the FP64 estimator on the same samples:
+    """FP32- (or BF16-) estimator error against the FP64 estimator on the same samples:
     per-quartet mean differences, overall and binned by the number of
     distinct nuclei of the quartet (the classes of src/compare_nuclei.F90)."""
     from slater_stoch_amd import basis as B
@@ -105,13 +105,13 @@ def precision_report(bas, m64, e64, m32):
     return out
 
 
-def water_precision(eng):
+def water_precision(eng, prec="fp32"):
     """The same report on the 7-AO water molecule (10 x 4000 samples)."""
     from slater_stoch_amd import configs
     w = configs.water_minimal()
     eng.set_basis(w)
     m64, e64 = eng.mc_blocks(NBL, PILOT, "fp64", seed=3, stream=0)
-    m32, _ = eng.mc_blocks(NBL, PILOT, "fp32", seed=3, stream=0)
+    m32, _ = eng.mc_blocks(NBL, PILOT, prec, seed=3, stream=0)
     return precision_report(w, m64, e64, m32)
 
 
@@ -352,7 +352,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-estimator variant")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32- and BF16-product estimator variants")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -388,19 +389,29 @@ def main():
               f"{r['analytic']:.2f} s, MC {r['mc']:.2f} s, {r['samples']} samples, max stderr "
               f"{r['max_stderr']:.2e})", file=sys.stderr, flush=True)
     t64 = float(np.mean([r["total"] for r in res64]))
-    res32 = None
+    res32 = res16 = None
     if not args.no_fp32:
         res32 = job(eng, bas, "fp32", seed=1)
         print(f"[bench] fp32 step: {res32['total']:.2f} s (MC {res32['mc']:.2f} s)",
               file=sys.stderr, flush=True)
+        res16 = job(eng, bas, "bf16", seed=1)
+        print(f"[bench] bf16 step: {res16['total']:.2f} s (MC {res16['mc']:.2f} s)",
+              file=sys.stderr, flush=True)
     last = res64[-1]
     roof, roof_other = roofline_of(last, bas)
-    precision = None
+    precision = precision16 = None
     if res32 is not None:
         # step 1 (seed 1) and the FP32 job share the pilot samples
         precision = {"c2": precision_report(bas, res64[0]["pilot"][0], res64[0]["pilot"][1],
                                             res32["pilot"][0]),
                      "c1_water": water_precision(eng)}
+        precision16 = {"c2": precision_report(bas, res64[0]["pilot"][0], res64[0]["pilot"][1],
+                                              res16["pilot"][0]),
+                       "c1_water": water_precision(eng, "bf16")}
+        print(f"[bench] bf16 vs fp64 estimator on the same samples: max |diff| "
+              f"{precision16['c2']['max_abs']:.2e} Ha (110 AOs, "
+              f"{precision16['c2']['max_abs_over_fp64_stderr']:.2f} of the FP64 stderr), "
+              f"{precision16['c1_water']['max_abs']:.2e} Ha (water)", file=sys.stderr, flush=True)
         print(f"[bench] fp32 vs fp64 estimator on the same samples: max |diff| "
               f"{precision['c2']['max_abs']:.2e} Ha (110 AOs), "
               f"{precision['c1_water']['max_abs']:.2e} Ha (water)", file=sys.stderr, flush=True)
@@ -437,6 +448,9 @@ def main():
         "mixed_precision_s": None if res32 is None else res32["total"],
         "mixed_precision_max_stderr": None if res32 is None else res32["max_stderr"],
         "mixed_precision_error": precision,
+        "bf16_products_s": None if res16 is None else res16["total"],
+        "bf16_products_max_stderr": None if res16 is None else res16["max_stderr"],
+        "bf16_products_error": precision16,
         "estimator_qsamples_per_s": (bas.nint * NBL * last["production_npts"] / (est_ms / 1e3)
                                      if est_ms > 0 else None),
         "mfma_fraction": 0.0,

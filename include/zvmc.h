@@ -49,6 +49,10 @@ extern "C" {
                                  bit-identical to the CPU oracle          */
 #define ZV_PREC_FP64 1        /* tiled FP64 estimator, split S-G form      */
 #define ZV_PREC_FP32 2        /* tiled FP32 estimator, FP64 block sums     */
+#define ZV_PREC_BF16 3        /* FP32 fields and kernel values; the two
+                                 products K A1 A2 + K B1 B2 from factors
+                                 rounded to BF16, summed by dot2 in FP32;
+                                 FP64 block sums                            */
 
 typedef struct zv_context zv_context;
 

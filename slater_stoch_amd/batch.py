@@ -129,7 +129,7 @@ def main(argv=None):
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--batch-seed", type=int, default=64)
     ap.add_argument("--max-ao", type=int, default=150)
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--target", type=float, default=1e-6)
     ap.add_argument("--workers", type=int, default=1,
                     help="must be 1: concurrent contexts are not supported")

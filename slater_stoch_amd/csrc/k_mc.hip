@@ -366,6 +366,14 @@ __global__ void k_dens_group(Basis B, PairTab Tb, int npp, const int *__restrict
 // stages cost more LDS than they gain in latency hiding, and a 16-sample FP32
 // stage holds more prefetch registers than two waves per SIMD allow (FP32
 // estimator 13.5 ms per block at 16, 11.8 ms at 8 on the 110-AO molecule)
+// two BF16 values in one register, for the dot2 instruction
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ inline bf16x2 bf16_pair(float x, float y)
+{
+    bf16x2 v = {(__bf16)x, (__bf16)y};
+    return v;
+}
+
 template <typename T> struct GroupSub;
 template <> struct GroupSub<float> { static constexpr int SUB = 8; };
 template <> struct GroupSub<double> { static constexpr int SUB = 8; };
@@ -378,7 +386,7 @@ template <> struct GroupSub<double> { static constexpr int SUB = 8; };
 // the reference's assignment (the electron-1 pair is the larger one), so
 // every quartet is written once.  Samples stream through LDS in stages of
 // SUB; stage sums in T are flushed to FP64.
-template <typename T, int NR, int NC, int TR, int TC, bool SPLIT>
+template <typename T, int NR, int NC, int TR, int TC, bool SPLIT, bool BF>
 __global__ __launch_bounds__(256) void k_est_group(
     const T *__restrict__ comp, const T *__restrict__ grp, const int *__restrict__ urow_pos,
     const int *__restrict__ urow_sp, int nurow, const int *__restrict__ ucol_pos,
@@ -544,7 +552,45 @@ __global__ __launch_bounds__(256) void k_est_group(
             // K scales the side with fewer products: the row components
             // (ER x TC per field) or the column components (TR x EC)
             constexpr bool SCALE_COLS = TR * EC < ER * TC;
-            if constexpr (!SCALE_COLS) {
+            if constexpr (BF) {
+                // BF16 products: both factors of K A1 A2 + K B1 B2 rounded to
+                // BF16 (round to nearest even) and the pair summed by one
+                // dot2 into the FP32 stage sum
+                static_assert(sizeof(T) == 4, "BF16 products only in the FP32 kernels");
+                if constexpr (!SCALE_COLS) {
+                    bf16x2 pr[ER][TC], pc[EC];
+#pragma unroll
+                    for (int i = 0; i < ER; i++)
+#pragma unroll
+                        for (int j = 0; j < TC; j++)
+                            pr[i][j] = bf16_pair(K[i / NR][j] * a1[i], K[i / NR][j] * b1[i]);
+#pragma unroll
+                    for (int j = 0; j < EC; j++) pc[j] = bf16_pair(a2[j], b2[j]);
+#pragma unroll
+                    for (int i = 0; i < ER; i++)
+#pragma unroll
+                        for (int j = 0; j < TC; j++)
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                a[i][j * NC + c] = __builtin_amdgcn_fdot2_f32_bf16(
+                                    pr[i][j], pc[j * NC + c], a[i][j * NC + c], false);
+                } else {
+                    bf16x2 pr[ER], pc[TR][EC];
+#pragma unroll
+                    for (int i = 0; i < ER; i++) pr[i] = bf16_pair(a1[i], b1[i]);
+#pragma unroll
+                    for (int u = 0; u < TR; u++)
+#pragma unroll
+                        for (int j = 0; j < EC; j++)
+                            pc[u][j] = bf16_pair(K[u][j / NC] * a2[j], K[u][j / NC] * b2[j]);
+#pragma unroll
+                    for (int i = 0; i < ER; i++)
+#pragma unroll
+                        for (int j = 0; j < EC; j++)
+                            a[i][j] = __builtin_amdgcn_fdot2_f32_bf16(pr[i], pc[i / NR][j],
+                                                                      a[i][j], false);
+                }
+            } else if constexpr (!SCALE_COLS) {
                 T ka[ER][TC], kb[ER][TC];
 #pragma unroll
                 for (int i = 0; i < ER; i++)
@@ -872,7 +918,8 @@ static int pick_split(zv_context *c, int Mc, int precision)
     int64_t tiles = 0;
     for (int k = 0; k < 9; k++) tiles += c->grp_ntile[k];
     if (tiles <= 0) return 1;
-    const int sub = precision == ZV_PREC_FP32 ? GroupSub<float>::SUB : GroupSub<double>::SUB;
+    const bool single = precision == ZV_PREC_FP32 || precision == ZV_PREC_BF16;
+    const int sub = single ? GroupSub<float>::SUB : GroupSub<double>::SUB;
     int64_t n = (1024 + tiles - 1) / tiles;
     n = std::min<int64_t>(n, 64);
     n = std::min<int64_t>(n, std::max<int64_t>(1, Mc / sub / 4));
@@ -957,7 +1004,7 @@ static int group_ready(zv_context *c)
     return ZV_OK;
 }
 
-template <typename T, int NR, int NC, int TR, int TC>
+template <typename T, bool BF, int NR, int NC, int TR, int TC>
 static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const T *comp,
                            const T *grp, const int *urp, const int *urs, int nur, const int *ucp,
                            const int *ucs, int nuc, const int2 *tiles, const int *pos_pair,
@@ -965,12 +1012,12 @@ static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const 
                            double *mpart)
 {
     if (nsplit > 1)
-        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, true>), dim3(ntile, (unsigned)nsplit),
+        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, true, BF>), dim3(ntile, (unsigned)nsplit),
                            dim3(256), 0, st, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles,
                            pos_pair, Mround, mspan, Mc, c->npair, c->grp_nsp_pad, acc, mpart,
                            c->nint);
     else
-        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, false>), dim3(ntile), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_est_group<T, NR, NC, TR, TC, false, BF>), dim3(ntile), dim3(256), 0, st,
                            comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, pos_pair, Mround,
                            mspan, Mc, c->npair, c->grp_nsp_pad, acc, mpart, c->nint);
 }
@@ -979,7 +1026,7 @@ static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const 
 // alone, so they run side by side on the main and the side streams (they
 // write disjoint quartets) and are joined back into the main stream.  With
 // nsplit > 1 every tile is also split over the samples of the chunk.
-template <typename T>
+template <typename T, bool BF>
 static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int Mround, int Mc,
                                int nsplit, double *acc, double *mpart)
 {
@@ -1010,7 +1057,7 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
             const int nur = c->grp_nunit[a], nuc = c->grp_nunit[b];
             const int2 *tiles = (const int2 *)g + c->grp_tile_off[3 * a + b];
 #define ZV_GROUP(NR, NC, TR, TC)                                                                  \
-    launch_group_t<T, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
+    launch_group_t<T, BF, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
                                       pos_pair, Mround, nsplit, mspan, Mc, acc, mpart)
             switch (3 * a + b) {
             case 0: ZV_GROUP(1, 1, 4, 4); break;
@@ -1041,7 +1088,8 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
     const int64_t ntri = c->nint;
     const int np = c->npair, npp = c->npair_pad;
     const int Mc = pick_chunk(c, npts, precision);
-    size_t fbytes = precision == ZV_PREC_FP32 ? 4 : 8;
+    const bool single = precision == ZV_PREC_FP32 || precision == ZV_PREC_BF16;
+    size_t fbytes = single ? 4 : 8;
     if (precision != ZV_PREC_FP64_PARITY) ZV_CHECK(group_ready(c));
     const size_t ncomp = (size_t)4 * Mc * np, ngrp = (size_t)6 * Mc * c->nsp;
     ZV_CHECK(zv_ensure(c, c->b_r1, sizeof(double) * 3 * Mc));
@@ -1076,7 +1124,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
         for (int64_t m0 = 0; m0 < npts; m0 += Mc) {
             int Mv = (int)((npts - m0) < Mc ? (npts - m0) : Mc);
             uint64_t g0 = (uint64_t)b * (uint64_t)npts + (uint64_t)m0;
-            int sub = precision == ZV_PREC_FP32 ? GroupSub<float>::SUB : GroupSub<double>::SUB;
+            int sub = single ? GroupSub<float>::SUB : GroupSub<double>::SUB;
             int Mround = (Mv + sub - 1) / sub * sub;
             if (mode == ZV_SAMPLES_FIXED) {
                 PhaseTimer tm(c, PH_COPY);
@@ -1109,7 +1157,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                 const int *g = (const int *)c->b_gmeta.p;
                 const int *pos_pair = g, *sp_first = g + c->grp_possp_off + np;
                 const dim3 grid(nblk(Mc, 256) * (unsigned)c->nsp);
-                if (precision == ZV_PREC_FP32) {
+                if (single) {
                     float *comp = (float *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
@@ -1119,7 +1167,12 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    ZV_CHECK(run_group_estimator<float>(c, comp, grp, Mround, Mc, nsplit, accS, mpart));
+                    if (precision == ZV_PREC_BF16)
+                        ZV_CHECK((run_group_estimator<float, true>(c, comp, grp, Mround, Mc,
+                                                                   nsplit, accS, mpart)));
+                    else
+                        ZV_CHECK((run_group_estimator<float, false>(c, comp, grp, Mround, Mc,
+                                                                    nsplit, accS, mpart)));
                 } else {
                     double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
@@ -1130,7 +1183,8 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
-                    ZV_CHECK(run_group_estimator<double>(c, comp, grp, Mround, Mc, nsplit, accS, mpart));
+                    ZV_CHECK((run_group_estimator<double, false>(c, comp, grp, Mround, Mc,
+                                                                 nsplit, accS, mpart)));
                 }
             }
         }

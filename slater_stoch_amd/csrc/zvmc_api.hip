@@ -520,7 +520,7 @@ extern "C" int zv_mc_blocks(zv_context *c, int nbl, int64_t npts, int mode, uint
         return zv_fail(c, ZV_ERR_ARG, "unknown sampling mode %d", mode);
     if (mode == ZV_SAMPLES_FIXED && (!r1 || !r2 || !pi0))
         return zv_fail(c, ZV_ERR_ARG, "fixed-sample mode needs r1, r2, pi0");
-    if (precision < ZV_PREC_FP64_PARITY || precision > ZV_PREC_FP32)
+    if (precision < ZV_PREC_FP64_PARITY || precision > ZV_PREC_BF16)
         return zv_fail(c, ZV_ERR_ARG, "unknown precision %d", precision);
     int rc = zv_run_mc(c, nbl, npts, mode, seed, stream, precision, r1, r2, pi0, mean, err,
                        block_means);

@@ -210,6 +210,7 @@ program two_e_slater_hip
   call get_environment_variable('ZVMC_PRECISION', env)
   if (trim(env) == 'fp64_parity') prec = ZV_PREC_FP64_PARITY
   if (trim(env) == 'fp32') prec = ZV_PREC_FP32
+  if (trim(env) == 'bf16') prec = ZV_PREC_BF16
   seed = 1 + (num_simulation-1)*10000     ! src/two_e_slater.F90:115
   print *, ' Number of Monte Carlo steps= ', nbl, ' X ', npts
   call zv_check(ctx, zv_mc_blocks(ctx, nbl, int(npts,c_int64_t), ZV_SAMPLES_PHILOX, seed, &

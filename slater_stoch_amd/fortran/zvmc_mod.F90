@@ -12,6 +12,7 @@ module zvmc
   integer(c_int), parameter :: ZV_OK = 0
   integer(c_int), parameter :: ZV_SAMPLES_FIXED = 0, ZV_SAMPLES_PHILOX = 1
   integer(c_int), parameter :: ZV_PREC_FP64_PARITY = 0, ZV_PREC_FP64 = 1, ZV_PREC_FP32 = 2
+  integer(c_int), parameter :: ZV_PREC_BF16 = 3
 
   interface
     integer(c_int) function zv_create(device, ctx) bind(C, name="zv_create")

@@ -19,8 +19,9 @@ LIB_PATH = Path(__file__).resolve().parent / "libzvmc.so"
 
 ZV_OK, ZV_ERR_ARG, ZV_ERR_HIP, ZV_ERR_STATE, ZV_ERR_NOMEM, ZV_ERR_NODEV = range(6)
 SAMPLES_FIXED, SAMPLES_PHILOX = 0, 1
-PREC_FP64_PARITY, PREC_FP64, PREC_FP32 = 0, 1, 2
-PRECISIONS = {"fp64_parity": PREC_FP64_PARITY, "fp64": PREC_FP64, "fp32": PREC_FP32}
+PREC_FP64_PARITY, PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2, 3
+PRECISIONS = {"fp64_parity": PREC_FP64_PARITY, "fp64": PREC_FP64, "fp32": PREC_FP32,
+              "bf16": PREC_BF16}
 PHASES = ["sample", "densities", "estimator", "statistics", "prim_pairs", "analytic_eri",
           "copies", "cleanup"]
 

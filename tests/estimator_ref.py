@@ -11,7 +11,10 @@ import numpy as np
 from slater_stoch_amd import basis as B
 
 
-def fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts):
+def fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts, split=False):
+    """With split, also returns sum_m (|w A1 A2| + |w B1 B2|) / (r12 M) per
+    quartet, the scale of the two products of the split form
+    t = w ((rho1 - rhoG1) rho2 + rhoG1 (rho2 - rhoG2)) / r12."""
     n = bas.n
     pairs = [(i, k) for i in range(1, n + 1) for k in range(1, i + 1)]
     M = r1.shape[0]
@@ -29,6 +32,7 @@ def fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts):
     Q = (js_ * (js_ - 1)) // 2 + ls_ - 1
     mean = np.zeros(P.size)
     scale = np.zeros(P.size)
+    scale2 = np.zeros(P.size)
     step = max(1, 200000 // M)
     for s in range(0, P.size, step):
         p, q = P[s:s + step], Q[s:s + step]
@@ -37,4 +41,7 @@ def fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts):
         t = w * (A[p, 0] * A[q, 1] - A[p, 2] * A[q, 3]) / r12
         mean[s:s + step] = t.reshape(-1, nbl, npts).mean(axis=2).mean(axis=1)
         scale[s:s + step] = np.abs(t).mean(axis=1)
-    return mean, scale
+        if split:
+            s2 = np.abs(w * (A[p, 0] - A[p, 2]) * A[q, 1]) + np.abs(w * A[p, 2] * (A[q, 1] - A[q, 3]))
+            scale2[s:s + step] = (s2 / r12).mean(axis=1)
+    return (mean, scale, scale2) if split else (mean, scale)

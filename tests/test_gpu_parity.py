@@ -85,12 +85,17 @@ def test_mc_fixed_parity_bitwise_all_orbital_types(engine, oracle):
     assert np.array_equal(err, oe)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5), ("fp64_parity", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5), ("fp64_parity", 2e-6),
+                                      ("bf16", 8e-3)])
 def test_estimators_match_fp64_reference(engine, oracle, prec, tol):
     """Every estimator against a cast-free FP64 numpy evaluation of the same
     formula on the same samples; the bound is tol * sum_m|t_m|/M per quartet
     (FP32 casts of the parity form: 2 x 2^-24 per sample; FP32 tiles: FP32
-    coordinates and hardware rsqrt)."""
+    coordinates and hardware rsqrt).  BF16 products: each factor of the two
+    products K A1 A2 and K B1 B2 is rounded to BF16 (relative 2^-9), so each
+    product is within (1 + 2^-9)^2 - 1 = 3.9e-3 of itself and the bound is
+    taken over sum_m (|K A1 A2| + |K B1 B2|) / M, with a factor 2 for the
+    FP32 parts."""
     from estimator_ref import fp64_estimate
     for bas, nbl, npts in ((configs.water_minimal(ng0=4), 4, 400), (_mixed_basis(), 2, 40)):
         engine.set_basis(bas)
@@ -98,7 +103,9 @@ def test_estimators_match_fp64_reference(engine, oracle, prec, tol):
         rng = np.random.default_rng(3)
         r1, r2 = rng.standard_normal((nbl * npts, 3)), rng.standard_normal((nbl * npts, 3))
         pi0 = oracle.pi0(r1, r2)
-        ref, scale = fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts)
+        ref, scale, scale2 = fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts, split=True)
+        if prec == "bf16":
+            scale = scale2
         got, _ = engine.mc_blocks(nbl, npts, prec, samples=(r1, r2, pi0))
         bad = np.abs(got - ref) > tol * scale + 1e-300
         assert not bad.any(), (prec, np.max(np.abs(got - ref) / scale))
