@@ -546,11 +546,20 @@ int plan_threads(size_t n)
     return (int)std::max(1u, std::min(8u, hw ? hw : 1u));
 }
 
-// run f(t) for t = 0 .. nt-1, on nt - 1 extra threads and this one
+// run f(t) for t = 0 .. nt-1, on nt - 1 extra threads and this one; a
+// range whose thread cannot be started runs here (f(t) are independent)
 template <typename F> void run_threads(int nt, F &&f)
 {
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(f, t);
+    int started = 1;
+    for (int t = 1; t < nt; t++, started++) {
+        try {
+            th.emplace_back(f, t);
+        } catch (...) {
+            break;
+        }
+    }
+    for (int t = started; t < nt; t++) f(t);
     f(0);
     for (auto &x : th) x.join();
 }
