@@ -17,8 +17,10 @@
  *    src/build_mapping_ijkl.F90:16-43 (kcp = 1..nint).
  *  - Every call returns 0 on success or a ZV_ERR_* code; zv_last_error()
  *    returns the message.  The Fortran host turns nonzero into `stop`.
- *  - One HIP stream per context (plus side streams joined back inside a
- *    call); calls are synchronous for the caller.  A context is not
+ *  - Contexts on a device share one HIP stream and four side streams,
+ *    made by the first context and kept until the process ends; side
+ *    streams are joined back inside a call, and calls are synchronous for
+ *    the caller.  A context is not
  *    thread-safe: drive it from one host thread.  Calls on different
  *    contexts from several threads run one at a time (a process-wide lock
  *    in every call that creates, drives or destroys a context): three

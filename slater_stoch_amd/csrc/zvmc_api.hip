@@ -4,6 +4,7 @@
 // k_clean.hip (PSD cleanup).
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "zvmc_internal.h"
@@ -179,6 +180,39 @@ extern "C" int zv_build_mapping(int n, int *is, int *js, int *ks, int *ls)
 }
 
 // ---------------------------------------------------------------- context
+// The streams of every context on a device: made by the first context and
+// kept until the process ends.  With streams of their own, contexts created
+// after the first in a process ran the estimator 9-15 % slower on MI355X
+// (their streams land on other hardware queues than the first context's);
+// calls run one at a time (call_lock) and each ends with its streams
+// drained, so contexts can share them.  Guarded by call_lock.
+struct DevStreams {
+    hipStream_t main = nullptr;
+    hipStream_t side[zv_context::NSIDE] = {};
+};
+static std::map<int, DevStreams> g_streams;
+
+static bool device_streams(int device, DevStreams &out)
+{
+    auto it = g_streams.find(device);
+    if (it != g_streams.end()) {
+        out = it->second;
+        return true;
+    }
+    DevStreams s;
+    bool ok = hipStreamCreate(&s.main) == hipSuccess;
+    for (int k = 0; ok && k < zv_context::NSIDE; k++) ok = hipStreamCreate(&s.side[k]) == hipSuccess;
+    if (!ok) {
+        if (s.main) (void)hipStreamDestroy(s.main);
+        for (hipStream_t x : s.side)
+            if (x) (void)hipStreamDestroy(x);
+        return false;
+    }
+    g_streams[device] = s;
+    out = s;
+    return true;
+}
+
 extern "C" int zv_create(int device, zv_context **out)
 {
     ZV_SERIALIZE;
@@ -192,11 +226,15 @@ extern "C" int zv_create(int device, zv_context **out)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ZV_ERR_NODEV;
     zv_context *c = new zv_context();
     c->device = device;
-    bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&c->stream) == hipSuccess &&
+    DevStreams ds;
+    bool ok = hipSetDevice(device) == hipSuccess && device_streams(device, ds) &&
               hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    if (ok) {
+        c->stream = ds.main;
+        for (int k = 0; k < zv_context::NSIDE; k++) c->side[k] = ds.side[k];
+    }
     for (int k = 0; ok && k < zv_context::NSIDE; k++)
-        ok = hipStreamCreate(&c->side[k]) == hipSuccess &&
-             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+        ok = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         zv_destroy(c);
         return ZV_ERR_HIP;
@@ -241,12 +279,10 @@ extern "C" int zv_destroy(zv_context *c)
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    for (int k = 0; k < zv_context::NSIDE; k++) {
-        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    // the streams belong to the device (device_streams), not to the context
+    for (int k = 0; k < zv_context::NSIDE; k++)
         if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
-    }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ZV_OK;
 }
