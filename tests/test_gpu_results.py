@@ -3,7 +3,8 @@
 A NULL host output keeps the per-quartet array on the device; fetched
 afterwards it carries the same bits as the copy the call would have made,
 and the on-device maximum equals numpy's over that array.  A new basis
-invalidates the kept results."""
+invalidates the kept results.  Kept results also show that the analytic
+ERIs repeat bit for bit (concurrent shell kernels, threaded planner)."""
 import numpy as np
 import pytest
 
@@ -65,3 +66,15 @@ def test_new_basis_invalidates_results(engine):
             engine.get_result(which)
         with pytest.raises(Z.ZVError):
             engine.result_max(which)
+
+
+def test_analytic_eris_repeat_bit_for_bit(engine):
+    """The shell kernels of a Schwarz phase run side by side on several
+    streams and their work plan is built on host threads; partial sums are
+    added in a fixed order, so two calls give the same bits."""
+    from slater_stoch_amd import zvmc as Z
+    engine.set_basis(configs.random_molecule())
+    engine.gauss_eri(device_only=True)
+    first = engine.get_result(Z.RESULT_ERI)
+    engine.gauss_eri(device_only=True)
+    np.testing.assert_array_equal(engine.get_result(Z.RESULT_ERI), first)
