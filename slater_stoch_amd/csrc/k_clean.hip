@@ -156,7 +156,7 @@ __global__ void k_colnormalize(int64_t N, double *X)
     for (int64_t t = threadIdx.x; t < N; t += 256) s += x[t] * x[t];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -173,7 +173,7 @@ __global__ void k_coldot(int64_t N, int r, const double *U, const double *Vt, do
     for (int64_t t = threadIdx.x; t < N; t += 256) s += U[t + (int64_t)k * N] * Vt[k + t * r];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -201,7 +201,7 @@ __global__ void k_shift_diag(double *G, int r, double f)
     for (int k = threadIdx.x; k < r; k += 256) s += G[(int64_t)k * r + k];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -224,7 +224,7 @@ __global__ void k_pivot_check(const double *G, int r, double thr, rocblas_int *f
     mn[threadIdx.x] = lo;
     mx[threadIdx.x] = hi;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
             mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + o]);
             mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
@@ -348,7 +348,7 @@ static int gram_tn(zv_context *c, rocblas_handle h, const double *A, const doubl
 static int orthonormalize(zv_context *c, rocblas_handle h, double *Z, int64_t N, int r,
                           const OrthoWork &w, int *n_fallback)
 {
-    const double one = 1.0, zero = 0.0;
+    const double one = 1.0;
     const double f = 11.0 * ((double)N * r + (double)r * (r + 1)) * 0x1.0p-53;
     ZV_HIPCHECK(c, hipMemcpyAsync(w.backup, Z, sizeof(double) * N * r, hipMemcpyDeviceToDevice,
                                   c->stream));

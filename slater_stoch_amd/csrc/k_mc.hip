@@ -790,7 +790,7 @@ __global__ void k_block_max(int64_t n, const double *x, double *part)
         m = fmax(m, x[t]);
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_one_e(OneEArgs A)
     red[1][threadIdx.x] = v;
     red[2][threadIdx.x] = kin;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (unsigned o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o)
             for (int r = 0; r < 3; r++) red[r][threadIdx.x] += red[r][threadIdx.x + o];
         __syncthreads();
