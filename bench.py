@@ -16,7 +16,9 @@ the ERIs, means and standard errors computed in HBM (the library keeps them
 there, and the sample-size policy reads the largest standard error reduced
 on the device); copying the three arrays to the host (447 MB over PCIe)
 comes after it and is reported as ``host_copy_s`` / ``with_host_copy_s``.  FP64 is the plain
-GPU form (``value``); the FP32 estimator variant is reported beside it.
+GPU form (``value``); the FP32 estimator and the BF16-product estimator
+variants are reported beside it, each with its error against the FP64
+estimator on the same samples.
 A bounded sample of the same job on the CPU oracle (all host cores) gives
 ``cpu_baseline``.  Prints one JSON line.
 """
