@@ -14,7 +14,8 @@
 // for every component u of U and l of L.
 //
 // Work item of one wavefront: a shell quartet, a range of lane-side
-// components, up to ZV_MAXU uniform-side components and a slice of the
+// components, a range of uniform-side components (up to 64 in the reduction
+// mode, at most kDirectMaxU in the direct mode) and a slice of the
 // uniform-side primitive pairs.  Lanes own lane-side primitive pairs (their
 // E in registers, reloaded per lane component); the uniform primitive pairs
 // are walked in the inner loop through an AoS copy ([a][component][h]) read
@@ -58,6 +59,13 @@ struct ShellGroup {
     int su, sl, cu0, nu, cl0, nl, ns, pad; // ns slices of [nu][nl] partials from part on
     int64_t part;
 };
+
+// Uniform components of a direct-mode item: the planner sends ranges of
+// kDirectMaxU + 1 or more uniform components to the reduction mode, so a
+// direct item holds a whole shell pair of at most this many components or
+// the short tail of a longer one (checked on the host before each launch).
+// The direct kernel's LDS accumulators are sized for it.
+constexpr int kDirectMaxU = 3;
 
 // AoS copies of a shell pair's primitive data and E (one block per shell pair)
 __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, double *eu,
@@ -114,7 +122,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                                                   double *__restrict__ part)
 {
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
-    __shared__ double accs[ZV_MAXU * CLB * 64];
+    __shared__ double accs[kDirectMaxU * CLB * 64]; // it.nu <= kDirectMaxU (host-checked)
     const int lane = threadIdx.x;
     const ShellItem it = items[blockIdx.x];
     const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
@@ -477,7 +485,7 @@ double x_flops(int LU, int LL) { return 2.0 * hc(LU) * hc(LL) + hc(LU); }
 
 // uniform components per item and the smallest count that uses the
 // reduction mode
-constexpr int kMaxUDirect = ZV_MAXU, kMaxURed = 64, kRedMin = 4;
+constexpr int kMaxUDirect = ZV_MAXU, kMaxURed = 64, kRedMin = kDirectMaxU + 1;
 
 // Estimated kernel cost of finishing, in lane flop-equivalents: the direct
 // mode per primitive quartet and pass (scalar-operand FMAs run about three
@@ -827,6 +835,13 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
     // Largest planned cost first, each to the stream with the least planned
     // cost so far.
     // bucket b = 25 v + 5 LU + LL, v = 2 (clb - 1) + red: statistics slot 36 + b
+    for (int b = 0; b < 100; b++) {
+        if ((b / 25) & 1) continue; // reduction mode
+        for (int t = pl.lo[b]; t < pl.lo[b + 1]; t++)
+            if (pl.items[t].nu < 1 || pl.items[t].nu > kDirectMaxU)
+                return zv_fail(c, ZV_ERR_STATE, "direct shell item with %d uniform components",
+                               pl.items[t].nu);
+    }
     constexpr int NST = 1 + zv_context::NSIDE;
     hipStream_t sts[NST] = {c->stream};
     for (int k = 0; k < zv_context::NSIDE; k++) sts[1 + k] = c->side[k];
