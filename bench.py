@@ -1,26 +1,31 @@
 #!/usr/bin/env python3
 """Time to a 1e-6 Ha standard error on every unique (ik|jl): the ZVMC ERI job.
 
-One step is the whole two-electron job of the reference's two_e_slater for a
-seeded 12-atom molecule (5 heavy atoms with 1S,2S,2S,2P,2P,3D; 7 H with
-1S,1S,2P; zeta ~ U[0.8, 8]; 110 STO orbitals, 6105 pairs, 18,638,565
-quartets; 20 Gaussians per STO fit):
+One step is the whole two-electron job of the reference's two_e_slater for
+the 200-orbital workload: 40 atoms (5 C, 35 H) at seeded positions in a 10 A
+box, C 1S,2S,2S,2P,3D and H 1S,2P with the reference's own example-basis
+exponents (example/VB1), 20100 pairs, 202,015,050 quartets, 20 Gaussians per
+STO fit:
 
   1. analytic Gaussian-fit ERIs with the Cauchy-Schwarz driver (FP64),
-  2. a pilot of 10 blocks x 4000 Monte Carlo samples,
-  3. if the largest standard error is above 1e-6 Ha, a production run of 10
-     blocks sized from the pilot (stderr ~ 1/sqrt(samples)).
+  2. a pilot of 10 blocks x 4000 Monte Carlo samples over every quartet,
+  3. while the largest standard error is above 1e-6 Ha: a further run of 10
+     blocks for the quartets above it only (zv_mc_refine: the estimator
+     tiles that hold them), sized from their error (stderr ~ 1/sqrt(samples)).
 
 The timed region starts with the basis resident on the device and ends with
-the ERIs, means and standard errors computed in HBM (the library keeps them
-there, and the sample-size policy reads the largest standard error reduced
-on the device); copying the three arrays to the host (447 MB over PCIe)
-comes after it and is reported as ``host_copy_s`` / ``with_host_copy_s``.  FP64 is the plain
-GPU form (``value``); the FP32 estimator and the BF16-product estimator
-variants are reported beside it, each with its error against the FP64
-estimator on the same samples.
-A bounded sample of the same job on the CPU oracle (all host cores) gives
-``cpu_baseline``.  Prints one JSON line.
+the ERIs, means and standard errors computed in HBM (the sample-size policy
+reads the largest standard error reduced on the device); copying the three
+arrays to the host comes after it (``host_copy_s`` / ``with_host_copy_s``).
+FP64 is the plain GPU form (``value``); the FP32 estimator job is reported
+beside it (``mixed_precision_s``) with its error against the FP64 estimator
+on the same samples.  Also reported: the 110-AO workload and water as extra
+keys, the configuration's Monte Carlo sample stream as a throughput, the
+cleanups of the pair supermatrix (randomized Rayleigh-Ritz of rank 2000 with
+an FP32 sketch, block Davidson for the 10 lowest eigenpairs), and the
+reference's CPU path (the reference build, oracle/_ref/libref.so) on a
+bounded sample of the same job, with the C port of the oracle beside it.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -35,21 +40,25 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+from slater_stoch_amd.shells import kernel_unit_pairs, shell_firsts  # noqa: E402
+
 METRIC = "sec to 1e-6 Ha stderr on all (ij|kl), mixed-prec vs FP64 GPU vs CPU; MFMA %"
 TARGET = 1e-6
 NBL = 10
 PILOT = 4000
+MAX_ROUNDS = 6
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
 
 
-def job(eng, bas, prec, seed):
-    """One step.  The timed region holds the device work of the job: the
-    per-quartet arrays stay in HBM (the sample-size policy reads the largest
-    standard error reduced on the device); fetching them to the host comes
-    after it and is reported separately (``host_copy_s``).  When a
-    production run follows the pilot, the pilot's arrays are fetched for the
-    precision report with the clock stopped."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def job(eng, bas, prec, seed, keep_pilot=False):
+    """One step.  The per-quartet arrays stay in HBM; with keep_pilot the
+    pilot's means and errors are fetched with the clock stopped (for the
+    precision report)."""
     from slater_stoch_amd.zvmc import RESULT_ERI, RESULT_MEAN, RESULT_STDERR
     eng.set_basis(bas)  # inputs resident; clears the primitive-pair cache
     t0 = time.perf_counter()
@@ -58,40 +67,53 @@ def job(eng, bas, prec, seed):
     eri_stats = eng.eri_stats()
     an_timings = eng.timings()
     eng.mc_blocks(NBL, PILOT, prec, seed=seed, stream=0, device_only=True)
-    mc_timings = eng.timings()
-    samples = NBL * PILOT
+    pilot_timings = eng.timings()
     emax = eng.result_max(RESULT_STDERR)
-    npts = PILOT
-    pilot, paused = None, 0.0
-    if emax > TARGET:
+    t2 = time.perf_counter()
+    pilot_emax = emax
+    paused = 0.0
+    pilot = None
+    if keep_pilot:
         tp = time.perf_counter()
         pilot = (eng.get_result(RESULT_MEAN), eng.get_result(RESULT_STDERR))
         paused = time.perf_counter() - tp
-        npts = int(math.ceil(PILOT * (emax / TARGET) ** 2 * 1.2 / 40.0)) * 40
-        npts = min(npts, 1_000_000)
-        eng.mc_blocks(NBL, npts, prec, seed=seed, stream=1, device_only=True)
-        mc_timings = eng.timings()
-        samples += NBL * npts
-        emax = eng.result_max(RESULT_STDERR)
-    t2 = time.perf_counter()
+    rounds = []
+    npts = PILOT
+    while emax > TARGET and len(rounds) < MAX_ROUNDS:
+        npts = min(int(math.ceil(npts * (emax / TARGET) ** 2 * 1.3 / 40.0)) * 40, 1_000_000)
+        nref, emax = eng.mc_refine(NBL, npts, prec, target=TARGET, seed=seed,
+                                   stream=1 + len(rounds))
+        tm = eng.timings()
+        rounds.append({"quartets": nref, "npts": npts, "max_stderr": emax,
+                       "estimator_ms": tm["estimator"][0], "densities_ms": tm["densities"][0]})
+    t3 = time.perf_counter()
+    tc = time.perf_counter()
     eri = eng.get_result(RESULT_ERI)
     mean, err = eng.get_result(RESULT_MEAN), eng.get_result(RESULT_STDERR)
-    t3 = time.perf_counter()
-    if pilot is None:
-        pilot = (mean, err)
-    return dict(total=t2 - t0 - paused, analytic=t1 - t0, mc=t2 - t1 - paused,
-                host_copy=t3 - t2, max_stderr=emax, samples=samples, production_npts=npts,
-                n_screened=nscr, eri_stats=eri_stats, an_timings=an_timings,
-                mc_timings=mc_timings, eri=eri, mean=mean, err=err, pilot=pilot)
+    host_copy = time.perf_counter() - tc
+    return dict(total=t3 - t0 - paused, analytic=t1 - t0, pilot=t2 - t1,
+                refine=t3 - t2 - paused, host_copy=host_copy, max_stderr=emax,
+                pilot_max_stderr=pilot_emax, rounds=rounds,
+                samples=NBL * PILOT + sum(NBL * r["npts"] for r in rounds),
+                n_screened=nscr, pre=pre, eri_stats=eri_stats, an_timings=an_timings,
+                pilot_timings=pilot_timings, eri=eri, mean=mean, err=err, pilot_arrays=pilot)
+
+
+def job_summary(r):
+    return {"total_s": r["total"], "analytic_s": r["analytic"], "pilot_s": r["pilot"],
+            "refine_s": r["refine"], "max_stderr": r["max_stderr"],
+            "pilot_max_stderr": r["pilot_max_stderr"], "refine_rounds": r["rounds"],
+            "n_screened": int(r["n_screened"])}
 
 
 def precision_report(bas, m64, e64, m32):
-    """FP32- (or BF16-) estimator error against the FP64 estimator on the same samples:
+    """FP32-estimator error against the FP64 estimator on the same samples:
     per-quartet mean differences, overall and binned by the number of
     distinct nuclei of the quartet (the classes of src/compare_nuclei.F90)."""
     from slater_stoch_amd import basis as B
     q = B.quartet_mapping(bas.n)
     cls = B.nuclei_class(bas, *q)
+    del q
     d = np.abs(m32 - m64)
     out = {"quartets": int(d.size), "max_abs": float(d.max()),
            "rms": float(np.sqrt(np.mean(d * d))),
@@ -107,14 +129,12 @@ def precision_report(bas, m64, e64, m32):
     return out
 
 
-def water_precision(eng, prec="fp32"):
-    """The same report on the 7-AO water molecule (10 x 4000 samples)."""
-    from slater_stoch_amd import configs
-    w = configs.water_minimal()
-    eng.set_basis(w)
+def same_sample_precision(eng, bas, prec):
+    """prec against FP64 on the same 10 x 4000 samples (a pilot run)."""
+    eng.set_basis(bas)
     m64, e64 = eng.mc_blocks(NBL, PILOT, "fp64", seed=3, stream=0)
     m32, _ = eng.mc_blocks(NBL, PILOT, prec, seed=3, stream=0)
-    return precision_report(w, m64, e64, m32)
+    return precision_report(bas, m64, e64, m32)
 
 
 def kernel_name(key):
@@ -125,39 +145,37 @@ def kernel_name(key):
     return f"k_eri_class<{a}, {b}>" if a < 5 else "k_eri_generic"
 
 
-def _traffic(name):
-    """HBM bytes per launch of a kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/kernel_traffic.json)."""
+def _traffic_table():
+    """HBM bytes per launch of each kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     tf = ROOT / "profiles" / "kernel_traffic.json"
     if not tf.exists():
-        return None
+        return {}
     try:
-        return json.loads(tf.read_text()).get(name)
-    except Exception:
-        return None
+        return json.loads(tf.read_text())
+    except ValueError:
+        return {}
 
 
 def roofline_eri(res):
-    """The analytic ERI kernels of the last step: the shell-kernel buckets of
-    one Schwarz phase run side by side on the library's five streams and are
-    timed together (HIP events around the fork and join), like the nine
-    estimator kernels of a block.  Algorithmic FP64 flops (Boys values and R
-    table once per primitive quartet, Hermite contractions per component, as
-    counted by the library) over the phase time; a "launch" is one phase's
-    set of buckets."""
+    """The analytic ERI kernels of the step: the shell-kernel buckets of one
+    Schwarz phase run side by side on the library's five streams and are
+    timed together (HIP events around the fork and join on the library's
+    stream).  Algorithmic FP64 flops (Boys values and R table once per
+    primitive quartet, Hermite contractions per component, as counted by the
+    library) over the phase time; a "launch" is one phase's set of buckets."""
     st = res["eri_stats"]
     flops = sum(v["flops"] for v in st.values())
     ms, nset = res["an_timings"]["analytic_eri"]
     achieved = flops / (ms / 1e3) / 1e12
     heavy = max(st.items(), key=lambda kv: kv[1]["flops"])
-    # HBM bytes: the committed per-kernel, per-launch traffic summed over the
-    # launches of the step (None unless every kernel is in the table)
+    table = _traffic_table()
     traffic = 0.0
     for key, d in st.items():
-        t = _traffic(kernel_name(key))
-        if t is None or traffic is None:
+        t = table.get(kernel_name(key))
+        if t is None:
             traffic = None
-            continue
+            break
         traffic += t * d["launches"]
     return {"bound": "fp64-valu",
             "kernel": "k_eri_shell<LU, LL, B> / k_eri_shell_red<LU, LL, B> (the buckets of a "
@@ -172,120 +190,51 @@ def roofline_eri(res):
             "heaviest_kernel_flop_share": heavy[1]["flops"] / flops}
 
 
-def shell_firsts(bas):
-    """First AO of every shell of the library (consecutive AOs with equal
-    centre, fit, type and order, at most 15), then n."""
-    n = bas.n
-    lao = bas.npower.sum(1) + 2 * (bas.i_type == 2)
-    first = []
-    for i in range(n):
-        same = (i > 0 and i - first[-1] < 15 and bas.i_type[i] == bas.i_type[i - 1]
-                and bas.n_gauss[i] == bas.n_gauss[i - 1] and lao[i] == lao[i - 1]
-                and np.array_equal(bas.center[i], bas.center[i - 1])
-                and np.array_equal(bas.g_gauss[i], bas.g_gauss[i - 1])
-                and np.array_equal(bas.c_gauss[:, i], bas.c_gauss[:, i - 1]))
-        if not same:
-            first.append(i)
-    first.append(n)
-    return first
-
-
-def shell_units(bas):
-    """The library's split of every shell pair's components into units of 6,
-    3 and 1 (csrc/k_mc.hip, group_ready): returns, per shell pair (I, K), the
-    first shell I and the list of units as arrays of pair indices."""
-    first = shell_firsts(bas)
-    out = []
-    for I in range(len(first) - 1):
-        for K in range(I + 1):
-            comps = [i * (i + 1) // 2 + k for i in range(first[I], first[I + 1])
-                     for k in range(first[K], first[K + 1]) if k <= i]
-            units, at = [], 0
-            for size in (6, 3, 1):
-                while len(comps) - at >= size:
-                    units.append(np.array(comps[at:at + size]))
-                    at += size
-            out.append((I, units))
-    return out
-
-
-def kernel_unit_pairs(bas):
-    """Unit pairs (row unit U, column unit V) holding at least one quartet
-    with row pair P >= column pair Q: the 1/r12 evaluations per sample."""
-    sps = shell_units(bas)
-    nsh = max(I for I, _ in sps) + 1
-    per_I = np.zeros(nsh)
-    for I, units in sps:
-        per_I[I] += len(units)
-    below = np.cumsum(per_I) - per_I           # units with first shell < I
-    total = float(np.sum(per_I * below))
-    by_I = {}
-    for I, units in sps:
-        by_I.setdefault(I, []).extend(units)
-    for units in by_I.values():                # same first shell: per unit pair
-        hi = np.array([u.max() for u in units])
-        lo = np.array([u.min() for u in units])
-        total += float(np.sum(hi[:, None] >= lo[None, :]))
-    return total
-
-
-def roofline_estimator(res, bas, prec):
-    """The group estimator of one block (9 class-pair kernels running side
-    by side, timed together with HIP events): algorithmic flops = 4 per
-    quartet-sample (two FMAs: K (A1 A2 + B1 B2)) + 8 per unit-pair-sample
-    (the three differences and r^2 of the kernel), and one reciprocal square
-    root per unit-pair-sample."""
-    ms, n = res["mc_timings"]["estimator"]
-    samples = NBL * res["production_npts"]
+def roofline_estimator(timings, bas, prec, samples):
+    """The group estimator over every quartet (the pilot): one "launch" is
+    the nine class-pair kernels of a chunk, timed together.  Algorithmic
+    flops = 4 per quartet-sample (two FMAs: K (A1 A2 + B1 B2)) + 8 per
+    unit-pair-sample (the three differences and r^2 of the kernel); one
+    reciprocal square root per unit-pair-sample."""
+    ms, n = timings["estimator"]
     nup = kernel_unit_pairs(bas)
     flops = (4.0 * bas.nint + 8.0 * nup) * samples
     achieved = flops / (ms / 1e3) / 1e12
-    peak = PEAK_FP64 if prec != "fp32" else PEAK_FP32
-    t = "double" if prec != "fp32" else "float"
-    traffic = None
-    tf = ROOT / "profiles" / "kernel_traffic.json"
-    if tf.exists():
-        try:
-            tj = json.loads(tf.read_text())
-            parts = [v for k, v in tj.items() if k.startswith(f"k_est_group<{t}")]
-            traffic = sum(parts) if parts else None
-        except Exception:
-            traffic = None
-    return {"bound": "fp64-valu" if prec != "fp32" else "fp32-valu",
-            "kernel": f"k_est_group<{t}, NR, NC, TR, TC> (9 class-pair launches per block)",
+    peak = PEAK_FP64 if prec == "fp64" else PEAK_FP32
+    t = "double" if prec == "fp64" else "float"
+    parts = [v for k, v in _traffic_table().items() if k.startswith(f"k_est_group<{t}")]
+    return {"bound": "fp64-valu" if prec == "fp64" else "fp32-valu",
+            "kernel": f"k_est_group<{t}, NR, NC, TR, TC> (9 class-pair launches per chunk)",
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": traffic, "launches": n, "total_ms": ms,
+            "traffic": sum(parts) if parts else None, "launches": n, "total_ms": ms,
             "avg_launch_ms": ms / max(1, n), "flops_per_launch": flops / max(1, n),
             "quartet_samples_per_s": float(bas.nint) * samples / (ms / 1e3),
             "kernel_evaluations_per_s": nup * samples / (ms / 1e3)}
 
 
-def density_report(res, bas, prec):
+def density_report(timings, bas, prec, samples):
     """The density kernel (one thread per shell pair and sample): per shell
     pair and sample it evaluates the STO and its fit for the first AO of both
     shells at both points -- 4 (1 + ng) exponentials -- and writes four
     component values per pair plus six point coordinates per shell pair."""
-    ms, n = res["mc_timings"]["densities"]
-    samples = NBL * res["production_npts"]
+    ms, n = timings["densities"]
     first = shell_firsts(bas)
     ng = [int(bas.n_gauss[first[I]]) for I in range(len(first) - 1)]
     nsh = len(ng)
     exps = sum(2 * ((1 + ng[I]) + (1 + ng[K])) for I in range(nsh) for K in range(I + 1))
     nsp = nsh * (nsh + 1) // 2
-    width = 4 if prec == "fp32" else 8
+    width = 8 if prec == "fp64" else 4
     written = width * (4 * bas.npair + 6 * nsp) * samples
     sec = ms / 1e3
-    return {"kernel": f"k_dens_group<{'float' if prec == 'fp32' else 'double'}>",
+    return {"kernel": f"k_dens_group<{'double' if prec == 'fp64' else 'float'}>",
             "launches": n, "total_ms": ms, "exp_per_s": exps * samples / sec,
             "bytes_written": written, "write_GBps": written / sec / 1e9,
             "write_frac_of_hbm_peak": written / sec / 8e12}
 
 
 def eri_by_class(res):
-    """Per (LU, LL) shell class of the analytic phase (both lane batches and
-    modes together): algorithmic FP64 flops, their share of the phase, and
-    primitive quartet x lane component evaluations; heaviest first.  The
-    classes run side by side, so the phase time is not split among them."""
+    """Per (LU, LL) shell class of the analytic phase: algorithmic FP64
+    flops and their share, heaviest first."""
     agg = {}
     for key, d in res["eri_stats"].items():
         k = (key[1], key[2]) if key[0] == "shell" else ("component", key[1], key[2])
@@ -298,54 +247,152 @@ def eri_by_class(res):
             for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["flops"])]
 
 
-def roofline_of(res, bas):
-    """The dominant launch of the step (largest total time): the estimator's
-    class-pair kernels of a block or the ERI buckets of a Schwarz phase, each
-    set timed together."""
-    eri, est = roofline_eri(res), roofline_estimator(res, bas, "fp64")
-    return (est, eri) if est["total_ms"] >= eri["total_ms"] else (eri, est)
+def sample_stream(eng, bas, npts):
+    """The configuration's Monte Carlo stream (1e8 samples in 1e6-sample
+    chunks, every quartet) as a throughput: one run of 10 blocks of npts
+    samples per precision, quartet-samples per second and the time 1e8
+    samples would take at that rate."""
+    eng.set_basis(bas)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        t = time.perf_counter()
+        eng.mc_blocks(NBL, npts, prec, seed=9, stream=9, device_only=True)
+        sec = time.perf_counter() - t
+        m = NBL * npts
+        out[prec] = {"samples": m, "seconds": sec,
+                     "quartet_samples_per_s": bas.nint * m / sec,
+                     "estimated_s_for_1e8_samples": sec * 1e8 / m,
+                     "estimator_ms": eng.timings()["estimator"][0]}
+    return out
 
 
-def cpu_baseline(bas, samples, nscr, budget_s=20.0):
-    """The CPU oracle (C restatement of the reference, OpenMP) on a bounded
-    sample of the same job, scaled to the whole job:
-      analytic: gauss_ijkl on a seeded random subset of quartets, scaled by the
-                number of unscreened quartets;
-      MC: one 40-sample block over all quartets, scaled by the job's samples."""
+def cleanups(eng, bas, eri_final):
+    """Cleanups of the pair supermatrix V[P][Q] = (ik|jl) (20100 x 20100,
+    3.2 GB) of the final ERIs (analytic + Monte Carlo mean): randomized
+    Rayleigh-Ritz of rank 2000 with 2 power iterations and an FP32 sketch,
+    then block Davidson for the 10 lowest eigenpairs, each timed alone."""
+    eng.set_basis(bas)
+    out = {}
+    t = time.perf_counter()
+    _, ritz, nrem = eng.rsvd_clean(eri_final, rank=2000, q=2, seed=1, sketch="fp32")
+    out["rsvd"] = {"rank": 2000, "power_iterations": 2, "sketch": "fp32",
+                   "seconds": time.perf_counter() - t, "negative_removed": nrem,
+                   "ritz_min": float(ritz.min()), "ritz_max": float(ritz.max())}
+    t = time.perf_counter()
+    _, ev, res, nrem, its = eng.davidson_lowest(eri_final, nev=10, tol=1e-7, max_iter=1000)
+    out["davidson"] = {"nev": 10, "tol": 1e-7, "max_iter": 1000,
+                       "seconds": time.perf_counter() - t, "iterations": its,
+                       "lowest": ev.tolist(), "residuals": res.tolist(),
+                       "converged": int((res <= 1e-7).sum()), "negative_removed": nrem}
+    return out
+
+
+def _random_canonical(rng, n, count):
+    """Random canonical quartets (i k | j l): pairs P = (i,k), Q = (j,l) drawn
+    uniformly, P >= Q; returned as (i, k, j, l)."""
+    npair = n * (n + 1) // 2
+    P = rng.integers(0, npair, count)
+    Q = rng.integers(0, npair, count)
+    P, Q = np.maximum(P, Q), np.minimum(P, Q)
+
+    def ik(p):
+        i = np.floor((np.sqrt(8.0 * p + 1.0) - 1.0) / 2.0).astype(np.int64) + 1
+        i = np.where(i * (i - 1) // 2 > p, i - 1, i)
+        i = np.where(i * (i + 1) // 2 <= p, i + 1, i)
+        return i, p - i * (i - 1) // 2 + 1
+    i, k = ik(P)
+    j, l = ik(Q)
+    return tuple(a.astype(np.int32) for a in (i, k, j, l))
+
+
+def cpu_baseline(bas, pre, nscr, pilot_emax, budget_s=24.0):
+    """The reference's CPU path on a bounded sample of the same job.
+
+    Analytic: the reference's gauss_ijkl (src/gauss_ijkl.F90, compiled from
+    its own sources into oracle/_ref/libref.so) on random canonical quartets
+    that pass the Schwarz test, one thread, until half the budget is used;
+    scaled by the number of unscreened quartets.
+    Monte Carlo: the reference's densities, Jacobians and estimator (the
+    harness's restatement of src/two_e_slater.F90:398-461 over the
+    reference's compute_densities) for one 40-sample walker block over two
+    random quartet subsets, fitted as a + b nq and scaled to every quartet
+    and to the samples the reference's own policy needs (one npts for all
+    quartets: the pilot's 4000 per block times (pilot max stderr / 1e-6)^2).
+    The oracle's C port (OpenMP, all host cores) is timed on the same
+    quartets beside it (``port``): the calibration ratio port / reference."""
     sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_lib import Oracle
-    from slater_stoch_amd import basis as B
+    from oracle_lib import Oracle, Reference
+    rng = np.random.default_rng(77)
+    n = bas.n
+    npts_ref = max(PILOT, int(math.ceil(PILOT * (pilot_emax / TARGET) ** 2 / 40.0)) * 40)
+    samples_ref = NBL * npts_ref
+    qi, qk, qj, ql = _random_canonical(rng, n, 400000)
+    live = np.abs(pre[qk - 1, qi - 1] * pre[ql - 1, qj - 1]) >= 1e-30
+    qi, qk, qj, ql = qi[live], qk[live], qj[live], ql[live]
+    try:
+        ref = Reference(bas)
+        kind = "reference"
+    except (FileNotFoundError, OSError):
+        ref = None
+        kind = "port"
     orc = Oracle()
     orc.set_basis(bas)
-    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-    q = B.quartet_mapping(bas.n)
-    nint = q[0].size
-    rng = np.random.default_rng(77)
-    # analytic: grow the subset until the time budget is half used
-    done, t_an, nsub = 0, 0.0, 0
-    perm = rng.permutation(nint)
-    chunk = 64
-    while t_an < budget_s / 2 and done < nint:
-        pick = perm[done:done + chunk]
-        t = time.perf_counter()
-        orc.gauss_eri([a[pick] for a in q], None)
-        t_an += time.perf_counter() - t
-        done += pick.size
-        chunk = min(4 * chunk, 4096)
-    an_est = t_an / done * (nint - nscr)
-    # MC: one block of 40 samples over every quartet
+    out = {"kind": kind, "samples_reference_policy": samples_ref}
+    done, t_an = 0, 0.0
+    if ref is not None:
+        t0 = time.perf_counter()
+        while done < qi.size:
+            ref.gauss_ijkl(qi[done], qk[done], qj[done], ql[done])
+            done += 1
+            if done % 16 == 0 and time.perf_counter() - t0 > budget_s / 2:
+                break
+        t_an = time.perf_counter() - t0
+    m = done if done else min(qi.size, 20000)
+    t0 = time.perf_counter()
+    orc.gauss_eri([qi[:m], qj[:m], qk[:m], ql[:m]], None)
+    t_port_an = time.perf_counter() - t0
     r = np.random.default_rng(5)
     r1, r2 = r.standard_normal((40, 3)), r.standard_normal((40, 3))
     pi0 = orc.pi0(r1, r2)
-    t = time.perf_counter()
-    orc.mc_fixed(1, 40, r1, r2, pi0, q)
-    t_mc = time.perf_counter() - t
-    mc_est = t_mc * samples / 40.0
-    return {"value": an_est + mc_est, "unit": "s", "cores": cores, "kind": "port",
-            "sample": (f"oracle gauss_ijkl on {done} random quartets ({t_an:.1f} s) scaled to "
-                       f"{nint - nscr} unscreened quartets; oracle estimator for all {nint} "
-                       f"quartets x 40 samples ({t_mc:.1f} s) scaled to {samples} samples"),
-            "analytic_s": an_est, "mc_s": mc_est}
+    nq1, nq2 = 20000, 200000
+    si, sk, sj, sl = _random_canonical(rng, n, nq2)
+    quartets = [si, sj, sk, sl]  # is, js, ks, ls
+
+    def fit(fn):
+        ts = []
+        for nq in (nq1, nq2):
+            t0 = time.perf_counter()
+            fn([a[:nq] for a in quartets])
+            ts.append(time.perf_counter() - t0)
+        b = (ts[1] - ts[0]) / (nq2 - nq1)
+        return max(ts[0] - b * nq1, 0.0), b
+    a_p, b_p = fit(lambda q: orc.mc_fixed(1, 40, r1, r2, pi0, q))
+    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    an_port = t_port_an / m * (bas.nint - nscr)
+    mc_port = (samples_ref / 40.0) * (a_p + b_p * bas.nint)
+    out["port"] = {"cores": cores, "analytic_s": an_port, "mc_s": mc_port,
+                   "value": an_port + mc_port}
+    if ref is not None:
+        a_r, b_r = fit(lambda q: ref.mc_fixed(1, 40, r1, r2, pi0, q))
+        an_ref = t_an / done * (bas.nint - nscr)
+        mc_ref = (samples_ref / 40.0) * (a_r + b_r * bas.nint)
+        out.update({"value": an_ref + mc_ref, "unit": "s", "cores": 1,
+                    "analytic_s": an_ref, "mc_s": mc_ref,
+                    "sample": (f"reference gauss_ijkl on {done} random unscreened quartets "
+                               f"({t_an:.1f} s, 1 thread) scaled to {bas.nint - nscr} "
+                               f"unscreened quartets; reference estimator on 40 samples over "
+                               f"{nq1} and {nq2} random quartets (fit a + b nq) scaled to "
+                               f"{bas.nint} quartets x {samples_ref} samples (the reference's "
+                               f"one-npts policy); the oracle's C port on the same quartets "
+                               f"in 'port' ({cores} threads)")})
+        out["port"]["ratio_to_reference"] = {
+            "analytic_per_quartet": (t_port_an / m) / (t_an / done),
+            "mc_per_quartet": b_p / b_r if b_r > 0 else None}
+    else:
+        out.update({"value": an_port + mc_port, "unit": "s", "cores": cores,
+                    "analytic_s": an_port, "mc_s": mc_port,
+                    "sample": f"oracle C port (reference build absent) on {m} quartets"})
+    return out
 
 
 def main():
@@ -354,8 +401,11 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-fp32", action="store_true",
-                    help="skip the FP32- and BF16-product estimator variants")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only: skip the FP32 job, the 110-AO and water reports, "
+                         "the sample stream and the cleanups")
+    ap.add_argument("--stream-npts", type=int, default=10000,
+                    help="samples per block of the sample-stream throughput run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -378,52 +428,58 @@ def main():
     from slater_stoch_amd import configs
     from slater_stoch_amd.zvmc import Engine
 
-    bas = configs.random_molecule()
+    bas = configs.large_ch()
     eng = Engine(0)
     for w in range(args.warmup):
-        job(eng, bas, "fp64", seed=1000 + w)
-        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        r = job(eng, bas, "fp64", seed=1000 + w)
+        log(f"warmup {w + 1}/{args.warmup}: {r['total']:.2f} s")
     res64 = []
     for s in range(args.steps):
-        res64.append(job(eng, bas, "fp64", seed=1 + s))
+        res64.append(job(eng, bas, "fp64", seed=1 + s, keep_pilot=(s == 0)))
         r = res64[-1]
-        print(f"[bench] fp64 step {s + 1}/{args.steps}: {r['total']:.2f} s (analytic "
-              f"{r['analytic']:.2f} s, MC {r['mc']:.2f} s, {r['samples']} samples, max stderr "
-              f"{r['max_stderr']:.2e})", file=sys.stderr, flush=True)
+        log(f"fp64 step {s + 1}/{args.steps}: {r['total']:.2f} s (analytic {r['analytic']:.2f} s, "
+            f"pilot {r['pilot']:.2f} s, refine {r['refine']:.2f} s over "
+            f"{[x['quartets'] for x in r['rounds']]} quartets, max stderr {r['max_stderr']:.2e})")
     t64 = float(np.mean([r["total"] for r in res64]))
-    res32 = res16 = None
-    if not args.no_fp32:
-        res32 = job(eng, bas, "fp32", seed=1)
-        print(f"[bench] fp32 step: {res32['total']:.2f} s (MC {res32['mc']:.2f} s)",
-              file=sys.stderr, flush=True)
-        res16 = job(eng, bas, "bf16", seed=1)
-        print(f"[bench] bf16 step: {res16['total']:.2f} s (MC {res16['mc']:.2f} s)",
-              file=sys.stderr, flush=True)
     last = res64[-1]
-    roof, roof_other = roofline_of(last, bas)
-    precision = precision16 = None
-    if res32 is not None:
+    extra = {}
+    if not args.no_extra:
+        res32 = job(eng, bas, "fp32", seed=1, keep_pilot=True)
+        log(f"fp32 step: {res32['total']:.2f} s (pilot {res32['pilot']:.2f} s, refine "
+            f"{res32['refine']:.2f} s)")
         # step 1 (seed 1) and the FP32 job share the pilot samples
-        precision = {"c2": precision_report(bas, res64[0]["pilot"][0], res64[0]["pilot"][1],
-                                            res32["pilot"][0]),
-                     "c1_water": water_precision(eng)}
-        precision16 = {"c2": precision_report(bas, res64[0]["pilot"][0], res64[0]["pilot"][1],
-                                              res16["pilot"][0]),
-                       "c1_water": water_precision(eng, "bf16")}
-        print(f"[bench] bf16 vs fp64 estimator on the same samples: max |diff| "
-              f"{precision16['c2']['max_abs']:.2e} Ha (110 AOs, "
-              f"{precision16['c2']['max_abs_over_fp64_stderr']:.2f} of the FP64 stderr), "
-              f"{precision16['c1_water']['max_abs']:.2e} Ha (water)", file=sys.stderr, flush=True)
-        print(f"[bench] fp32 vs fp64 estimator on the same samples: max |diff| "
-              f"{precision['c2']['max_abs']:.2e} Ha (110 AOs), "
-              f"{precision['c1_water']['max_abs']:.2e} Ha (water)", file=sys.stderr, flush=True)
+        prec_c5 = precision_report(bas, res64[0]["pilot_arrays"][0],
+                                   res64[0]["pilot_arrays"][1], res32["pilot_arrays"][0])
+        log(f"fp32 vs fp64 on the same samples: max |diff| {prec_c5['max_abs']:.2e} Ha "
+            f"({prec_c5['max_abs_over_fp64_stderr']:.3f} of the FP64 stderr)")
+        c2 = configs.random_molecule()
+        c2_64 = job(eng, c2, "fp64", seed=1)
+        c2_32 = job(eng, c2, "fp32", seed=1)
+        log(f"110 AO: fp64 {c2_64['total']:.2f} s, fp32 {c2_32['total']:.2f} s")
+        water = configs.water_minimal()
+        extra = {
+            "mixed_precision_s": res32["total"],
+            "mixed_precision_max_stderr": res32["max_stderr"],
+            "mixed_precision_refine_rounds": res32["rounds"],
+            "mixed_precision_error": {
+                "c5": prec_c5,
+                "c2": same_sample_precision(eng, c2, "fp32"),
+                "c1_water": same_sample_precision(eng, water, "fp32")},
+            "bf16_products_error": {"c2": same_sample_precision(eng, c2, "bf16"),
+                                    "c1_water": same_sample_precision(eng, water, "bf16")},
+            "c2_110ao": {"workload": "random 12-atom molecule (5 C/N/O + 7 H), 110 STO AOs, "
+                                     "18638565 quartets", "fp64": job_summary(c2_64),
+                         "fp32": job_summary(c2_32), "roofline": roofline_eri(c2_64)},
+            "sample_stream": sample_stream(eng, bas, args.stream_npts),
+        }
+        log(f"sample stream: {extra['sample_stream']}")
+        ft = time.perf_counter()
+        extra["cleanup"] = cleanups(eng, bas, last["eri"] + last["mean"])
+        log(f"cleanups {time.perf_counter() - ft:.1f} s: {extra['cleanup']}")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bas, last["samples"], last["n_screened"])
-        print(f"[bench] cpu baseline estimate {cpu['value']:.0f} s on {cpu['cores']} cores",
-              file=sys.stderr, flush=True)
-    mct = last["mc_timings"]
-    est_ms = mct["estimator"][0]
+        cpu = cpu_baseline(bas, last["pre"], last["n_screened"], last["pilot_max_stderr"])
+        log(f"cpu baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.4g} s")
     line = {
         "metric": METRIC,
         "value": t64,
@@ -436,32 +492,32 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random molecule, device Philox samples)",
-        "config": {"workload": "random 12-atom molecule (5 C/N/O + 7 H), 110 STO AOs, "
-                               "6105 pairs, 18638565 quartets, 20-Gaussian fits; "
-                               "analytic FP64 ERIs + ZVMC to max stderr <= 1e-6 Ha, 10 blocks",
+        "data": "synthetic (seeded 40-atom C/H geometry, reference example exponents, "
+                "device Philox samples)",
+        "config": {"workload": "40-atom C/H molecule (5 C: 1S,2S,2S,2P,3D; 35 H: 1S,2P; "
+                               "example/VB1 exponents), 200 STO AOs, 20100 pairs, "
+                               "202015050 quartets, 20-Gaussian fits; analytic FP64 ERIs + "
+                               "ZVMC to max stderr <= 1e-6 Ha (10-block pilot, then targeted "
+                               "refinement)",
                    "nint": int(bas.nint), "npair": int(bas.npair), "nbasis": int(bas.n),
-                   "blocks": NBL, "samples": int(last["samples"]),
+                   "blocks": NBL, "pilot_samples": NBL * PILOT, "samples": int(last["samples"]),
                    "max_stderr": last["max_stderr"], "estimator": "fp64"},
         "analytic_s": float(np.mean([r["analytic"] for r in res64])),
+        "pilot_s": float(np.mean([r["pilot"] for r in res64])),
+        "refine_s": float(np.mean([r["refine"] for r in res64])),
+        "refine_rounds": last["rounds"],
+        "n_screened": int(last["n_screened"]),
         "host_copy_s": float(np.mean([r["host_copy"] for r in res64])),
         "with_host_copy_s": float(np.mean([r["total"] + r["host_copy"] for r in res64])),
-        "mc_s": float(np.mean([r["mc"] for r in res64])),
-        "mixed_precision_s": None if res32 is None else res32["total"],
-        "mixed_precision_max_stderr": None if res32 is None else res32["max_stderr"],
-        "mixed_precision_error": precision,
-        "bf16_products_s": None if res16 is None else res16["total"],
-        "bf16_products_max_stderr": None if res16 is None else res16["max_stderr"],
-        "bf16_products_error": precision16,
-        "estimator_qsamples_per_s": (bas.nint * NBL * last["production_npts"] / (est_ms / 1e3)
-                                     if est_ms > 0 else None),
         "mfma_fraction": 0.0,
-        "roofline": roof,
-        "roofline_other_phase": roof_other,
-        "density_phase": density_report(last, bas, "fp64"),
+        "roofline": roofline_eri(last),
+        "roofline_other_phase": roofline_estimator(last["pilot_timings"], bas, "fp64",
+                                                   NBL * PILOT),
+        "density_phase": density_report(last["pilot_timings"], bas, "fp64", NBL * PILOT),
         "eri_by_class": eri_by_class(last)[:12],
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     if cpu:
         line["speedup_vs_cpu"] = cpu["value"] / t64
     print(json.dumps(line))

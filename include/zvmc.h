@@ -146,6 +146,23 @@ int zv_mc_blocks(zv_context *ctx, int nbl, int64_t npts, int mode, uint64_t seed
 int zv_get_result(zv_context *ctx, int which, double *host);
 int zv_result_max(zv_context *ctx, int which, double *value);
 
+/* Targeted refinement of device-resident Monte Carlo results (no
+ * counterpart in the reference, which gives every quartet the same npts,
+ * src/two_e_slater.F90:148): the quartets whose kept standard error exceeds
+ * `target` get a fresh run of nbl (>= 2) blocks of npts Philox samples from
+ * (seed, stream) -- the group estimator restricted to the tiles that hold
+ * them, with the densities and block statistics of
+ * src/two_e_slater.F90:386-496 -- and their kept mean and standard error are
+ * replaced by that run's; every other quartet keeps its values.  Needs the
+ * ZV_RESULT_MEAN / ZV_RESULT_STDERR results of a zv_mc_blocks call on the
+ * current basis.  precision: ZV_PREC_FP64, ZV_PREC_FP32 or ZV_PREC_BF16.
+ * n_refined (out, may be NULL): quartets replaced; max_err (out, may be
+ * NULL): the largest kept standard error afterwards.  A time-to-target
+ * policy: one pilot zv_mc_blocks, then zv_mc_refine until max_err <= target,
+ * each round with more samples and another stream. */
+int zv_mc_refine(zv_context *ctx, int nbl, int64_t npts, uint64_t seed, uint64_t stream,
+                 int precision, double target, int64_t *n_refined, double *max_err);
+
 /* Samples drawn by the device sampler for global sample indices
  * m0 .. m0+m-1 (ZV_SAMPLES_PHILOX), copied to the host: r1(3,m), r2(3,m),
  * pi0(m).  src/draw_configuration.F90:1-29 + src/compute_jacobian_and_pi0.F90:8-17. */
@@ -170,6 +187,44 @@ int zv_svd_clean(zv_context *ctx, int rank, int q, uint64_t seed, double *eri,
  * same matrix, src/davidson_clean.F90:30-120 (dense branch). */
 int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
 
+/* Cleanups of the pair-packed ERI supermatrix V[P][Q] = (ik|jl), P = (i,k),
+ * Q = (j,l) the AO pairs i >= k, j >= l (N = n(n+1)/2, pair index
+ * i(i-1)/2 + k), built on the device from eri(kcp).  V is positive
+ * semidefinite in exact arithmetic; Monte Carlo noise leaves small negative
+ * eigenvalues, which both calls remove (eri in/out, written back as
+ * (V[P][Q] + V[Q][P]) / 2).
+ *
+ * zv_rsvd_clean: randomized range finder of rank min(rank, N) with q power
+ * iterations (src/randomized_svd.F90:22-65: Gaussian test matrix from
+ * `seed`, Z = V P, then Z <- V (V^T Z) with re-orthonormalization each
+ * iteration), followed by an FP64 Rayleigh-Ritz step in place of the SVD
+ * of src/svd_clean.F90:159-281 (B = Q^T V Q, symmetric eigensolver); the
+ * negative Ritz pairs are removed, V -= lambda (Q y)(Q y)^T.  `sketch`
+ * selects the precision of the range finder's N x N x r products:
+ * ZV_SKETCH_FP32 (SGEMM on an FP32 copy of V) or ZV_SKETCH_FP64; the
+ * orthonormalizations and the Rayleigh-Ritz step are FP64.  ritz (out, may
+ * be NULL): the min(rank, N) Ritz values, ascending; n_removed (out, may be
+ * NULL): the negative pairs removed, those converged in the sense of
+ * src/svd_clean.F90:286-296 (|u.v| >= 0.99, for a symmetric matrix a
+ * residual ||V x - lambda x|| <= sqrt(1 - 0.99^2) |lambda|). */
+#define ZV_SKETCH_FP64 0
+#define ZV_SKETCH_FP32 1
+int zv_rsvd_clean(zv_context *ctx, int rank, int q, uint64_t seed, int sketch, double *eri,
+                  double *ritz, int *n_removed);
+
+/* Block Davidson for the nev (1..64) lowest eigenpairs of the same V (the
+ * role of src/davidson.F90:1-159 in the iterative branch of
+ * src/davidson_clean.F90:120-323): Jacobi-preconditioned corrections, an
+ * orthonormal search space of up to max(8 nev, 48) vectors with thick
+ * restart, FP64 throughout.  Stops when every residual norm
+ * ||V x - theta x|| <= tol (Hartree) or after max_iter iterations; the
+ * negative eigenpairs whose residual is within tol are removed from V
+ * (eri in/out).  evals (out, nev, may be NULL): the Ritz values ascending;
+ * resid (out, nev, may be NULL): their residual norms; n_removed, iters
+ * (out, may be NULL). */
+int zv_davidson_lowest(zv_context *ctx, int nev, double tol, int max_iter, double *eri,
+                       double *evals, double *resid, int *n_removed, int *iters);
+
 /* Progress lines of the long-running calls, written to stdout: level 1
  * prints what the reference prints unconditionally in svd_clean ("Vectors
  * a -- b / N" per randomized pass, src/svd_clean.F90:153-154, and the
@@ -178,6 +233,30 @@ int zv_davidson_clean(zv_context *ctx, double *eri, int *n_removed);
  * the line reports work done); level 0 (the default) prints nothing.  No
  * counterpart in the reference, whose prints cannot be switched off. */
 int zv_set_verbose(zv_context *ctx, int level);
+
+/* Tuning knobs (0 = the library's automatic choice, the default).  No
+ * counterpart in the reference (whose only knobs are the OpenMP schedule and
+ * the input deck); for autotuners and for tests that pin each code path.
+ *  ZV_TUNE_EST_SPLIT  sample splits per estimator tile (1 = unsplit; capped
+ *                     at 64, at 4 LDS stages per split and at 1 GiB of
+ *                     partial sums)
+ *  ZV_TUNE_MC_CHUNK   largest number of samples per estimator chunk (rounded
+ *                     down to a multiple of 160, at least 160)
+ *  ZV_TUNE_ERI_ITEMS  the analytic shell planner cuts a Schwarz phase into
+ *                     work items of at most 1/value of its cost
+ *  ZV_TUNE_ERI_STREAMS streams that run the analytic kernel buckets side by
+ *                     side (1..5; 1 runs them one after another, which
+ *                     times every bucket on its own)
+ * Values < 0 give ZV_ERR_ARG; the setting holds for the context's later
+ * calls and changes only speed, except ZV_TUNE_EST_SPLIT and
+ * ZV_TUNE_MC_CHUNK, which also change the estimator's summation order
+ * (rounding only). */
+#define ZV_TUNE_EST_SPLIT 0
+#define ZV_TUNE_MC_CHUNK 1
+#define ZV_TUNE_ERI_ITEMS 2
+#define ZV_TUNE_ERI_STREAMS 3
+#define ZV_TUNE_N 8
+int zv_set_tuning(zv_context *ctx, int key, int64_t value);
 
 /* Milliseconds spent in the last call's kernels, by phase:
  * t[0] sampler, t[1] densities, t[2] estimator, t[3] block statistics,

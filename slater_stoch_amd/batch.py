@@ -8,15 +8,19 @@ by the block formula (src/two_e_slater.F90:488-496).  The reference runs one
 molecule with a fixed npts_two_elec; the sample-size policy and the batch
 are this package's:
 
-* a pilot of nbl x pilot_npts samples;
+* a pilot of nbl x pilot_npts samples over every quartet;
 * while the largest standard error exceeds the target and the sample budget
-  allows, a fresh run of nbl blocks with npts scaled from the latest run,
+  allows, a further run of nbl blocks with npts scaled from the latest run,
   npts' = npts (err_max / target)^2 x safety, rounded up to a multiple of
   nw = 40 and capped by what is left of max_samples (the estimator's error is
-  heavy-tailed, so one production run sized from the pilot can fall short);
-* the reported mean and standard error are those of the last run: one
-  complete nbl-block estimate, as two_e_slater writes it.  ``samples``
-  counts every run.
+  heavy-tailed, so one run sized from the pilot can fall short).  With the
+  default policy "refine" that run covers only the quartets still above the
+  target (zv_mc_refine: the estimator tiles that hold them), which replace
+  their values with the new run's; with "rerun" it covers every quartet
+  (the earlier policy, for comparison);
+* every reported mean and standard error is one complete nbl-block estimate
+  of a single run, as two_e_slater writes it.  ``samples`` counts the
+  samples of every run (a refinement run's samples serve its quartets only).
 
 Molecules run largest first through one Engine (one library context).
 Results are deterministic: molecule t uses Philox seed ``seed + t`` and
@@ -56,32 +60,43 @@ def next_npts(npts, err_max, target, safety, cap_npts):
 
 
 def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=4000,
-                 max_samples=10_000_000, safety=1.2, max_rounds=4, seed=0, keep_arrays=False):
-    """One molecule: analytic ERIs, pilot, then production runs by the policy
+                 max_samples=10_000_000, safety=1.2, max_rounds=4, seed=0, keep_arrays=False,
+                 policy="refine"):
+    """One molecule: analytic ERIs, pilot, then further runs by the policy
     above.  Returns a dict of the run (arrays only with keep_arrays)."""
+    from .zvmc import RESULT_ERI, RESULT_MEAN, RESULT_STDERR
+    if policy not in ("refine", "rerun"):
+        raise ValueError(f"unknown policy {policy!r}")
     t0 = time.perf_counter()
     eng.set_basis(bas)
-    eri, _, nscr = eng.gauss_eri()
+    _, _, nscr = eng.gauss_eri(device_only=True)
     t1 = time.perf_counter()
-    npts, rounds, used = pilot_npts, 0, 0
-    mean, err = eng.mc_blocks(nbl, npts, precision, seed=seed, stream=0)
+    npts, rounds, used, refined = pilot_npts, 0, 0, 0
+    eng.mc_blocks(nbl, npts, precision, seed=seed, stream=0, device_only=True)
+    emax = eng.result_max(RESULT_STDERR)
     used += nbl * npts
     rounds += 1
     while rounds < max_rounds:
-        emax = float(err.max()) if err.size else 0.0
         nxt = next_npts(npts, emax, target, safety, (max_samples - used) // nbl)
         if not nxt:
             break
         npts = nxt
-        mean, err = eng.mc_blocks(nbl, npts, precision, seed=seed, stream=rounds)
+        if policy == "refine":
+            nref, emax = eng.mc_refine(nbl, npts, precision, target=target, seed=seed,
+                                       stream=rounds)
+            refined += nref
+        else:
+            eng.mc_blocks(nbl, npts, precision, seed=seed, stream=rounds, device_only=True)
+            emax = eng.result_max(RESULT_STDERR)
         used += nbl * npts
         rounds += 1
     t2 = time.perf_counter()
-    emax = float(err.max()) if err.size else 0.0
     out = dict(n=int(bas.n), nint=int(bas.nint), n_screened=int(nscr), samples=int(used),
                rounds=rounds, npts=int(npts), max_stderr=emax, reached=bool(emax <= target),
-               t_analytic=t1 - t0, t_mc=t2 - t1)
+               refined_quartets=int(refined), t_analytic=t1 - t0, t_mc=t2 - t1)
     if keep_arrays:
+        eri = eng.get_result(RESULT_ERI)
+        mean, err = eng.get_result(RESULT_MEAN), eng.get_result(RESULT_STDERR)
         out.update(eri_gauss=eri, mc_mean=mean, stderr=err, eri=eri + mean)
     return out
 
@@ -135,6 +150,7 @@ def main(argv=None):
                     help="must be 1: concurrent contexts are not supported")
     ap.add_argument("--max-samples", type=int, default=10_000_000)
     ap.add_argument("--max-rounds", type=int, default=4)
+    ap.add_argument("--policy", default="refine", choices=["refine", "rerun"])
     ap.add_argument("--verbose", type=int, default=2,
                     help="engine progress level (2: one line per Monte Carlo block)")
     args = ap.parse_args(argv)
@@ -148,9 +164,10 @@ def main(argv=None):
     t0 = time.perf_counter()
     res = run_batch(bases, target=args.target, precision=args.precision, workers=args.workers,
                     callback=show, verbose=args.verbose, max_samples=args.max_samples,
-                    max_rounds=args.max_rounds)
+                    max_rounds=args.max_rounds, policy=args.policy)
     s = summary(res, time.perf_counter() - t0)
-    s.update(precision=args.precision, workers=args.workers, target=args.target)
+    s.update(precision=args.precision, workers=args.workers, target=args.target,
+             policy=args.policy)
     print(json.dumps(s), flush=True)
     return 0
 

@@ -5,7 +5,8 @@
   1S,2S,2S,2P,2P,3D and H gets 1S,1S,2P (double zeta + polarisation);
   zeta ~ U[0.8, 8] per (element, shell) because the reference basis file is
   per element (src/read_basis.F90:28-36).  12 atoms / 110 AOs by default.
-* ``large_ch``        -- 40 C/H atoms in a 10 Angstrom box, 200 AOs up to l = 2
+* ``large_ch``        -- 40 C/H atoms in a 10 Angstrom box, 200 AOs up to l = 2,
+  with the C and H exponents of the reference's example basis
 * ``batch``           -- 64 random molecules of uneven size
 
 Every generator returns a ``Basis`` (see basis.py) and can write the
@@ -62,12 +63,25 @@ def random_molecule(seed=2024, n_heavy=5, n_h=7, box=6.0, dmin=0.9, zmin=0.8, zm
     return expand_basis(geom, basis, ng0=ng0, a_zv=a_zv)
 
 
-def large_ch(seed=40, ng0=20, a_zv=1.8) -> Basis:
-    """40 atoms (5 C, 35 H) in a 10 A box: C 1S,2S,2S,2P,3D (12 AOs), H 1S,2P
-    (4 AOs) -> 200 AOs, 20100 pairs."""
-    return random_molecule(seed=seed, n_heavy=5, n_h=35, box=10.0, heavy=("C",),
-                           heavy_shells=["1S", "2S", "2S", "2P", "3D"], h_shells=["1S", "2P"],
-                           ng0=ng0, a_zv=a_zv)
+# C and H shells of the reference's own example basis (example/VB1:2,4,40,42-43,45,48):
+# the valence-range exponents a production basis uses, so that the 200-AO
+# workload is screened and sampled like a real molecule (compact exponents
+# drawn at random would let the Schwarz test drop almost every quartet).
+LARGE_CH_BASIS = {"C": [("1S", 5.206173), ("2S", 2.057774), ("2S", 1.329176), ("2P", 1.748226),
+                        ("3D", 1.953933)],
+                  "H": [("1S", 1.165255), ("2P", 1.481937)]}
+
+
+def large_ch(seed=40, ng0=20, a_zv=1.8, n_c=5, n_h=35, box=10.0, dmin=0.9) -> Basis:
+    """40 atoms (5 C, 35 H) at seeded positions in a 10 A box (min separation
+    0.9 A): C 1S,2S,2S,2P,3D (12 AOs), H 1S,2P (4 AOs) -> 200 AOs, 20100
+    pairs, with the exponents of ``LARGE_CH_BASIS``."""
+    rng = np.random.default_rng(seed)
+    labels = ["C"] * n_c + ["H"] * n_h
+    labels = [labels[t] for t in rng.permutation(len(labels))]
+    coords = _place_atoms(rng, len(labels), box, dmin)
+    geom = geometry_from_angstrom(labels, coords)
+    return expand_basis(geom, LARGE_CH_BASIS, ng0=ng0, a_zv=a_zv)
 
 
 def batch(seed=64, count=64, ng0=20, a_zv=1.8, max_ao=150):

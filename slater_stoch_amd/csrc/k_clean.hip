@@ -561,3 +561,465 @@ int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed)
     if (n_removed) *n_removed = l;
     return ZV_OK;
 }
+
+// ====================================================================== pair supermatrix
+// The pair-packed ERI supermatrix V[P + Q N] = (ik|jl), P = (i,k), Q = (j,l)
+// AO pairs (i >= k, j >= l, lexicographic, N = n(n+1)/2), symmetric and
+// positive semidefinite in exact arithmetic (1/r12 is a positive definite
+// kernel); Monte Carlo noise leaves small negative eigenvalues.  Two GPU
+// cleanups act on it:
+//
+// zv_rsvd_clean: randomized range finder of rank r with q power iterations
+// (src/randomized_svd.F90:22-65: Gaussian test matrix, Z = V P, Z <- V (V^T Z)
+// with re-orthonormalization), then Rayleigh-Ritz in FP64 (B = Q^T V Q,
+// symmetric eigendecomposition) instead of the SVD of Q^T V, so that the sign
+// of every Ritz value is explicit; negative Ritz pairs are removed,
+// V -= sum lambda (Q y)(Q y)^T.  The sketch products (the N x N x r GEMMs of
+// the range finder) run in FP32 when asked (SGEMM on an FP32 copy of V), the
+// orthonormalizations and the Rayleigh-Ritz step in FP64.
+//
+// zv_davidson_lowest: block Davidson for the nev lowest eigenpairs of V
+// (the role of src/davidson.F90:1-159 in src/davidson_clean.F90:120-323):
+// Ritz vectors of the search space, residuals R = V X - X Theta; the
+// residuals of the unconverged Ritz vectors, orthogonalized against the
+// space and orthonormalized (CholeskyQR3, Householder fallback), extend it;
+// thick restart to the current Ritz vectors when the space is full.  Every
+// projection keeps the basis orthonormal, so the small problem is a standard
+// symmetric one (the reference's generalized one lost definiteness).  No
+// diagonal (Jacobi) preconditioner: the wanted eigenvalues lie near zero
+// while diag V = (P|P) spans orders of magnitude, so R / (diag V - theta)
+// damps exactly the components the wanted vectors carry (on a noisy 30-AO
+// matrix it had not converged after 2000 iterations; without it, 167).  The
+// converged negative pairs are removed from V.
+namespace {
+
+// V from the quartet list: the kcp entry of (P, Q) at V[P,Q] and V[Q,P]
+__global__ void k_build_V(int64_t ntri, const double *eri, double *V, int64_t N, int n,
+                          const int *pi, const int *pk, const int64_t *kbase)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    int i, j, k, l;
+    const int64_t kcp = quartet_ref(t, pi, pk, kbase, n, i, j, k, l);
+    int64_t P, Q;
+    tri_decode(t, P, Q);
+    const double v = eri[kcp];
+    V[P + Q * N] = v;
+    V[Q + P * N] = v;
+}
+
+// eri[kcp] = (V[P,Q] + V[Q,P]) / 2
+__global__ void k_gather_V(int64_t ntri, const double *V, double *eri, int64_t N, int n,
+                           const int *pi, const int *pk, const int64_t *kbase)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    int i, j, k, l;
+    const int64_t kcp = quartet_ref(t, pi, pk, kbase, n, i, j, k, l);
+    int64_t P, Q;
+    tri_decode(t, P, Q);
+    eri[kcp] = 0.5 * (V[P + Q * N] + V[Q + P * N]);
+}
+
+__global__ void k_to_f32(int64_t n, const double *x, float *y)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = (float)x[t];
+}
+
+__global__ void k_to_f64(int64_t n, const float *x, double *y)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = (double)x[t];
+}
+
+// B = (B + B^T) / 2 for an r x r matrix (leading dimension r)
+__global__ void k_symmetrize(int r, double *B)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)r * r) return;
+    const int i = (int)(t % r), j = (int)(t / r);
+    if (i >= j) return;
+    const double m = 0.5 * (B[i + (int64_t)j * r] + B[j + (int64_t)i * r]);
+    B[i + (int64_t)j * r] = m;
+    B[j + (int64_t)i * r] = m;
+}
+
+// R = AX - X diag(theta), one thread per entry
+__global__ void k_residual(int64_t N, int k, const double *AX, const double *X,
+                           const double *theta, double *R)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * k) return;
+    const int j = (int)(t / N);
+    R[t] = AX[t] - X[t] * theta[j];
+}
+
+// out[j] = ||X(:,j)||, one workgroup per column
+__global__ void k_colnorms(int64_t N, const double *X, double *out)
+{
+    __shared__ double red[256];
+    const double *x = X + (int64_t)blockIdx.x * N;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < N; t += 256) s += x[t] * x[t];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (unsigned o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0]);
+}
+
+// fill the lower triangle of the m x m leading block of H (ld) from its upper one
+__global__ void k_mirror(int m, int ld, double *H)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m * m) return;
+    const int i = (int)(t % m), j = (int)(t / m);
+    if (i > j) H[i + (int64_t)j * ld] = H[j + (int64_t)i * ld];
+}
+
+} // namespace
+
+// V (N x N, device) from the host quartet list (kcp order)
+static int build_V(zv_context *c, const double *eri_host, double *d_eri, double *V, int64_t N)
+{
+    ZV_HIPCHECK(c, hipMemcpyAsync(d_eri, eri_host, sizeof(double) * c->nint,
+                                  hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_build_V, dim3(nblk(c->nint, 256)), dim3(256), 0, c->stream, c->nint,
+                       (const double *)d_eri, V, N, c->n, c->d_pair_i, c->d_pair_k, c->d_kbase);
+    ZV_HIPCHECK(c, hipGetLastError());
+    return ZV_OK;
+}
+
+static int gather_V(zv_context *c, const double *V, double *d_eri, double *eri_host, int64_t N)
+{
+    hipLaunchKernelGGL(k_gather_V, dim3(nblk(c->nint, 256)), dim3(256), 0, c->stream, c->nint, V,
+                       d_eri, N, c->n, c->d_pair_i, c->d_pair_k, c->d_kbase);
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_HIPCHECK(c, hipMemcpyAsync(eri_host, d_eri, sizeof(double) * c->nint,
+                                  hipMemcpyDeviceToHost, c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ZV_OK;
+}
+
+// V -= X diag(lam) X^T for the first m columns of X (N x m) with lam[j] < 0
+// (scaled copy in Xs)
+static int remove_pairs(zv_context *c, rocblas_handle h, double *V, int64_t N, const double *X,
+                        const double *d_lam, int m, double *Xs)
+{
+    if (m <= 0) return ZV_OK;
+    const double one = 1.0, mone = -1.0;
+    hipLaunchKernelGGL(k_colscale, dim3(nblk(N * m, 256)), dim3(256), 0, c->stream, N, m, X,
+                       d_lam, Xs, 1);
+    ZV_HIPCHECK(c, hipGetLastError());
+    RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)N,
+                        (rocblas_int)N, m, &mone, Xs, (rocblas_int)N, X, (rocblas_int)N, &one, V,
+                        (rocblas_int)N));
+    return ZV_OK;
+}
+
+int zv_run_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int sketch32, double *eri,
+                      double *ritz, int *n_removed)
+{
+    const int64_t N = c->npair;
+    const int r = (int)std::min<int64_t>(rank, N);
+    rocblas_handle h;
+    ZV_CHECK(blas_handle(c, &h));
+    PhaseTimer tm(c, PH_CLEAN);
+    const size_t nn = (size_t)N * N, nr = (size_t)N * r, rr = (size_t)r * r;
+    // V, its FP32 copy (sketch32), Z, AQ, backup (N r), FP32 panels (2 N r
+    // floats = N r doubles), B, (r^2), lam, E, tau (r), the quartet list, info
+    const size_t words = nn + (sketch32 ? nn / 2 + 1 : 0) + 4 * nr + rr + 3 * (size_t)r +
+                         (size_t)c->nint + 16;
+    ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
+    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * (kGramSplit + 1) * rr));
+    double *V = (double *)c->b_misc.p;
+    float *V32 = (float *)(V + nn);
+    double *Z = V + nn + (sketch32 ? nn / 2 + 1 : 0), *AQ = Z + nr, *bk = AQ + nr;
+    float *F1 = (float *)(bk + nr), *F2 = F1 + nr;
+    double *B = bk + 2 * nr, *lam = B + rr, *E = lam + r, *tau = E + r, *d_eri = tau + r;
+    rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    double *part = (double *)c->b_part.p;
+    const OrthoWork ow{B, bk, tau, info + 1, part};
+    int n_fallback = 0;
+    ZV_CHECK(build_V(c, eri, d_eri, V, N));
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    hipLaunchKernelGGL(k_randn, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, (int64_t)nr, seed,
+                       0u, AQ);
+    hipLaunchKernelGGL(k_colnormalize, dim3(r), dim3(256), 0, c->stream, N, AQ);
+    ZV_HIPCHECK(c, hipGetLastError());
+    // Z = V P, then q times Z <- V (V Z) (V symmetric: V^T = V), each product
+    // pair followed by an FP64 orthonormalization
+    if (sketch32) {
+        hipLaunchKernelGGL(k_to_f32, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, (int64_t)nn,
+                           (const double *)V, V32);
+        hipLaunchKernelGGL(k_to_f32, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, (int64_t)nr,
+                           (const double *)AQ, F1);
+        RB(c, rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, r,
+                            (rocblas_int)N, &onef, V32, (rocblas_int)N, F1, (rocblas_int)N,
+                            &zerof, F2, (rocblas_int)N));
+        hipLaunchKernelGGL(k_to_f64, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, (int64_t)nr,
+                           (const float *)F2, Z);
+    } else {
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, r,
+                            (rocblas_int)N, &one, V, (rocblas_int)N, AQ, (rocblas_int)N, &zero, Z,
+                            (rocblas_int)N));
+    }
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_CHECK(orthonormalize(c, h, Z, N, r, ow, &n_fallback));
+    for (int it = 0; it < q; it++) {
+        if (sketch32) {
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(nr, 256)), dim3(256), 0, c->stream,
+                               (int64_t)nr, (const double *)Z, F1);
+            RB(c, rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                r, (rocblas_int)N, &onef, V32, (rocblas_int)N, F1, (rocblas_int)N,
+                                &zerof, F2, (rocblas_int)N));
+            RB(c, rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                r, (rocblas_int)N, &onef, V32, (rocblas_int)N, F2, (rocblas_int)N,
+                                &zerof, F1, (rocblas_int)N));
+            hipLaunchKernelGGL(k_to_f64, dim3(nblk(nr, 256)), dim3(256), 0, c->stream,
+                               (int64_t)nr, (const float *)F1, Z);
+        } else {
+            RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                r, (rocblas_int)N, &one, V, (rocblas_int)N, Z, (rocblas_int)N,
+                                &zero, AQ, (rocblas_int)N));
+            RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                r, (rocblas_int)N, &one, V, (rocblas_int)N, AQ, (rocblas_int)N,
+                                &zero, Z, (rocblas_int)N));
+        }
+        ZV_HIPCHECK(c, hipGetLastError());
+        ZV_CHECK(orthonormalize(c, h, Z, N, r, ow, &n_fallback));
+    }
+    // Rayleigh-Ritz in FP64: B = Z^T V Z, B = Y diag(lam) Y^T (ascending)
+    RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, r,
+                        (rocblas_int)N, &one, V, (rocblas_int)N, Z, (rocblas_int)N, &zero, AQ,
+                        (rocblas_int)N));
+    ZV_CHECK(gram_tn(c, h, Z, AQ, N, r, B, part));
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblk((int64_t)rr, 256)), dim3(256), 0, c->stream, r, B);
+    ZV_HIPCHECK(c, hipGetLastError());
+    RB(c, rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, r, B, r, lam, E, info));
+    std::vector<double> hlam(r);
+    rocblas_int hinfo = 0;
+    ZV_HIPCHECK(c, hipMemcpyAsync(hlam.data(), lam, sizeof(double) * r, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (hinfo != 0) return zv_fail(c, ZV_ERR_HIP, "rsvd_clean: eigensolver info %d", (int)hinfo);
+    int neg = 0;
+    while (neg < r && hlam[neg] < 0.0) neg++;
+    int removed = 0;
+    if (neg > 0) {
+        // Ritz vectors of the negative values, X = Z Y(:, 0:neg) in AQ, and
+        // their residuals ||V x - lambda x||: a pair is removed only when
+        // converged in the sense of src/svd_clean.F90:286-296, |u.v| >= 0.99
+        // for its singular vectors, which for a symmetric matrix is a
+        // residual of at most sqrt(1 - 0.99^2) |lambda|
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, neg,
+                            r, &one, Z, (rocblas_int)N, B, r, &zero, AQ, (rocblas_int)N));
+        double *VX = Z; // Z is no longer needed
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, neg,
+                            (rocblas_int)N, &one, V, (rocblas_int)N, AQ, (rocblas_int)N, &zero, VX,
+                            (rocblas_int)N));
+        hipLaunchKernelGGL(k_residual, dim3(nblk((int64_t)N * neg, 256)), dim3(256), 0,
+                           c->stream, N, neg, (const double *)VX, (const double *)AQ,
+                           (const double *)lam, bk);
+        hipLaunchKernelGGL(k_colnorms, dim3(neg), dim3(256), 0, c->stream, N, (const double *)bk,
+                           E);
+        ZV_HIPCHECK(c, hipGetLastError());
+        std::vector<double> hres(neg);
+        ZV_HIPCHECK(c, hipMemcpyAsync(hres.data(), E, sizeof(double) * neg, hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+        // accepted pairs first (columns moved in place, values in lam)
+        const double accept = std::sqrt(1.0 - 0.99 * 0.99);
+        std::vector<double> hacc;
+        for (int j = 0; j < neg; j++) {
+            if (!(hres[j] <= accept * std::fabs(hlam[j]))) continue;
+            if (removed != j)
+                ZV_HIPCHECK(c, hipMemcpyAsync(AQ + (size_t)removed * N, AQ + (size_t)j * N,
+                                              sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                              c->stream));
+            hacc.push_back(hlam[j]);
+            removed++;
+        }
+        if (removed > 0) {
+            ZV_HIPCHECK(c, hipMemcpyAsync(E, hacc.data(), sizeof(double) * removed,
+                                          hipMemcpyHostToDevice, c->stream));
+            ZV_CHECK(remove_pairs(c, h, V, N, AQ, E, removed, bk));
+        }
+    }
+    if (c->verbose) {
+        printf(" rsvd_clean: rank %d, %d power iterations (%s sketch), Ritz values %g .. %g, "
+               "%d negative, %d converged and removed\n",
+               r, q, sketch32 ? "FP32" : "FP64", hlam[0], hlam[r - 1], neg, removed);
+        fflush(stdout);
+    }
+    if (ritz) std::copy(hlam.begin(), hlam.end(), ritz);
+    if (n_removed) *n_removed = removed;
+    return gather_V(c, V, d_eri, eri, N);
+}
+
+int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
+                           double *evals, double *resid, int *n_removed, int *iters)
+{
+    const int64_t N = c->npair;
+    const int k = nev;
+    // search space of up to 8 blocks (at least 48 vectors), never beyond N
+    const int mmax = (int)std::min<int64_t>(N, std::max(8 * k, 48));
+    if (k < 1 || 2 * k > mmax)
+        return zv_fail(c, ZV_ERR_ARG, "davidson: nev %d out of range for %lld pairs", nev,
+                       (long long)N);
+    rocblas_handle h;
+    ZV_CHECK(blas_handle(c, &h));
+    PhaseTimer tm(c, PH_CLEAN);
+    const size_t nn = (size_t)N * N, nm = (size_t)N * mmax, nk = (size_t)N * k;
+    const size_t mm = (size_t)mmax * mmax;
+    // V, S and AS (N mmax), X, AX, R, T, AT, bk (N k), H, Hw (mmax^2), C
+    // (mmax k), theta, E, tau, norms (mmax), G (k^2), the quartet list, info
+    const size_t words = nn + 2 * nm + 6 * nk + 2 * mm + (size_t)mmax * k + 4 * (size_t)mmax +
+                         (size_t)k * k + (size_t)c->nint + 16;
+    ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
+    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * (kGramSplit + 1) * (size_t)k * k));
+    double *V = (double *)c->b_misc.p, *S = V + nn, *AS = S + nm, *X = AS + nm, *AX = X + nk,
+           *R = AX + nk, *T = R + nk, *AT = T + nk, *bk = AT + nk, *H = bk + nk,
+           *Hw = H + mm, *C = Hw + mm, *theta = C + (size_t)mmax * k, *E = theta + mmax,
+           *tau = E + mmax, *nrm = tau + mmax, *G = nrm + mmax, *d_eri = G + (size_t)k * k;
+    rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    double *part = (double *)c->b_part.p;
+    const OrthoWork ow{G, bk, tau, info + 1, part};
+    int n_fallback = 0;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    ZV_CHECK(build_V(c, eri, d_eri, V, N));
+    // start: k Gaussian vectors, orthonormalized
+    hipLaunchKernelGGL(k_randn, dim3(nblk(nk, 256)), dim3(256), 0, c->stream, (int64_t)nk,
+                       (uint64_t)0x0DA71D50u, 0u, S);
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_CHECK(orthonormalize(c, h, S, N, k, ow, &n_fallback));
+    RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, k,
+                        (rocblas_int)N, &one, V, (rocblas_int)N, S, (rocblas_int)N, &zero, AS,
+                        (rocblas_int)N));
+    RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, k, k,
+                        (rocblas_int)N, &one, S, (rocblas_int)N, AS, (rocblas_int)N, &zero, H,
+                        mmax));
+    int m = k, it = 0;
+    std::vector<double> htheta(mmax), hnrm(k);
+    bool converged = false;
+    for (;;) {
+        // Ritz pairs of the space: Hw = H(0:m, 0:m) (upper triangle valid)
+        hipLaunchKernelGGL(k_mirror, dim3(nblk((int64_t)m * m, 256)), dim3(256), 0, c->stream, m,
+                           mmax, H);
+        ZV_HIPCHECK(c, hipGetLastError());
+        ZV_HIPCHECK(c, hipMemcpy2DAsync(Hw, sizeof(double) * m, H, sizeof(double) * mmax,
+                                        sizeof(double) * m, m, hipMemcpyDeviceToDevice,
+                                        c->stream));
+        RB(c, rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, m, Hw, m, theta, E,
+                               info));
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, k,
+                            m, &one, S, (rocblas_int)N, Hw, m, &zero, X, (rocblas_int)N));
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, k,
+                            m, &one, AS, (rocblas_int)N, Hw, m, &zero, AX, (rocblas_int)N));
+        hipLaunchKernelGGL(k_residual, dim3(nblk(nk, 256)), dim3(256), 0, c->stream, N, k,
+                           (const double *)AX, (const double *)X, (const double *)theta, R);
+        hipLaunchKernelGGL(k_colnorms, dim3(k), dim3(256), 0, c->stream, N, (const double *)R,
+                           nrm);
+        ZV_HIPCHECK(c, hipGetLastError());
+        rocblas_int hinfo = 0;
+        ZV_HIPCHECK(c, hipMemcpyAsync(htheta.data(), theta, sizeof(double) * k,
+                                      hipMemcpyDeviceToHost, c->stream));
+        ZV_HIPCHECK(c, hipMemcpyAsync(hnrm.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (hinfo != 0) return zv_fail(c, ZV_ERR_HIP, "davidson: eigensolver info %d", (int)hinfo);
+        double rmax = 0.0;
+        for (int j = 0; j < k; j++) rmax = std::max(rmax, hnrm[j]);
+        if (c->verbose > 1) {
+            printf(" davidson iteration %d: space %d, lowest %.6e, max residual %.3e\n", it, m,
+                   htheta[0], rmax);
+            fflush(stdout);
+        }
+        if (rmax <= tol) {
+            converged = true;
+            break;
+        }
+        if (it >= max_iter) break;
+        it++;
+        // the residuals of the unconverged Ritz vectors extend the space
+        int nt = 0;
+        for (int j = 0; j < k; j++)
+            if (hnrm[j] > tol) {
+                ZV_HIPCHECK(c, hipMemcpyAsync(T + (size_t)nt * N, R + (size_t)j * N,
+                                              sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                              c->stream));
+                nt++;
+            }
+        if (m + nt > mmax) {
+            // thick restart: the Ritz vectors span the new space (orthonormal)
+            ZV_HIPCHECK(c, hipMemcpyAsync(S, X, sizeof(double) * nk, hipMemcpyDeviceToDevice,
+                                          c->stream));
+            ZV_HIPCHECK(c, hipMemcpyAsync(AS, AX, sizeof(double) * nk, hipMemcpyDeviceToDevice,
+                                          c->stream));
+            std::vector<double> hH((size_t)k * mmax, 0.0);
+            for (int j = 0; j < k; j++) hH[(size_t)j * mmax + j] = htheta[j];
+            ZV_HIPCHECK(c, hipMemcpyAsync(H, hH.data(), sizeof(double) * k * mmax,
+                                          hipMemcpyHostToDevice, c->stream));
+            m = k;
+        }
+        // T <- T - S (S^T T), twice, an orthonormal basis of T, once more
+        for (int pass = 0; pass < 3; pass++) {
+            RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nt,
+                                (rocblas_int)N, &one, S, (rocblas_int)N, T, (rocblas_int)N, &zero,
+                                C, m));
+            RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                nt, m, &mone, S, (rocblas_int)N, C, m, &one, T, (rocblas_int)N));
+            if (pass > 0) ZV_CHECK(orthonormalize(c, h, T, N, nt, ow, &n_fallback));
+        }
+        RB(c, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N, nt,
+                            (rocblas_int)N, &one, V, (rocblas_int)N, T, (rocblas_int)N, &zero, AT,
+                            (rocblas_int)N));
+        // new columns of H: S^T AT above, T^T AT on the diagonal block
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nt,
+                            (rocblas_int)N, &one, S, (rocblas_int)N, AT, (rocblas_int)N, &zero,
+                            H + (size_t)m * mmax, mmax));
+        RB(c, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nt, nt,
+                            (rocblas_int)N, &one, T, (rocblas_int)N, AT, (rocblas_int)N, &zero,
+                            H + (size_t)m * mmax + m, mmax));
+        ZV_HIPCHECK(c, hipMemcpyAsync(S + (size_t)m * N, T, sizeof(double) * N * nt,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        ZV_HIPCHECK(c, hipMemcpyAsync(AS + (size_t)m * N, AT, sizeof(double) * N * nt,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        m += nt;
+    }
+    // the converged negative pairs are removed (columns moved to the front)
+    int neg = 0;
+    std::vector<double> hneg;
+    for (int j = 0; j < k; j++) {
+        if (!(htheta[j] < 0.0 && hnrm[j] <= tol)) continue;
+        if (neg != j)
+            ZV_HIPCHECK(c, hipMemcpyAsync(X + (size_t)neg * N, X + (size_t)j * N,
+                                          sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
+        hneg.push_back(htheta[j]);
+        neg++;
+    }
+    if (neg > 0) {
+        ZV_HIPCHECK(c, hipMemcpyAsync(E, hneg.data(), sizeof(double) * neg, hipMemcpyHostToDevice,
+                                      c->stream));
+        ZV_CHECK(remove_pairs(c, h, V, N, X, E, neg, bk));
+    }
+    if (c->verbose) {
+        printf(" davidson: %d lowest eigenvalues of the pair supermatrix, %s after %d "
+               "iterations, lowest %g, %d negative removed\n",
+               k, converged ? "converged" : "not all converged", it, htheta[0], neg);
+        fflush(stdout);
+    }
+    if (evals) std::copy(htheta.begin(), htheta.begin() + k, evals);
+    if (resid) std::copy(hnrm.begin(), hnrm.end(), resid);
+    if (n_removed) *n_removed = neg;
+    if (iters) *iters = it;
+    return gather_V(c, V, d_eri, eri, N);
+}

@@ -669,7 +669,9 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
     for (const Orient &o : ors) total += o.cost;
     // items of at most 1/40000 of the total cost: larger quartets are split,
     // so the sorted queue ends on short items
-    const double budget = std::max(total / 40000.0, 2.0e6);
+    const double nitems =
+        c->tune[ZV_TUNE_ERI_ITEMS] > 0 ? (double)c->tune[ZV_TUNE_ERI_ITEMS] : 40000.0;
+    const double budget = std::max(total / nitems, 2.0e6);
     std::vector<PlanPart> parts(nt);
     run_threads(nt, [&](int t) {
         const size_t q0 = nq * t / nt, q1 = nq * (t + 1) / nt;
@@ -863,8 +865,12 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
         const int cls = 36 + b;
         c->cls_pq[cls] += pl.units[b];
         c->cls_flops[cls] += pl.flops[b];
+        // ZV_TUNE_ERI_STREAMS limits the streams (1: buckets one after another)
+        const int nst = c->tune[ZV_TUNE_ERI_STREAMS] > 0
+                            ? (int)std::min<int64_t>(c->tune[ZV_TUNE_ERI_STREAMS], NST)
+                            : NST;
         int s = 0;
-        for (int q = 1; q < NST; q++)
+        for (int q = 1; q < nst; q++)
             if (load[q] < load[s]) s = q;
         load[s] += pl.flops[b];
         PhaseTimer tm(c, PH_N, cls, sts[s]);

@@ -900,6 +900,7 @@ static int pick_chunk(zv_context *c, int64_t npts, int precision)
     // of every LDS stage depth (8, 16, 32), so a rounded-up stage never
     // reads past the chunk
     int64_t mc = (int64_t)(budget / per);
+    if (c->tune[ZV_TUNE_MC_CHUNK] > 0) mc = std::min<int64_t>(mc, c->tune[ZV_TUNE_MC_CHUNK]);
     mc = mc / 160 * 160;
     if (mc < 160) mc = 160;
     if (mc > 65440) mc = 65440;
@@ -913,14 +914,15 @@ static int pick_chunk(zv_context *c, int64_t npts, int precision)
 // stages are split over up to 64 workgroups per tile, aiming at about 1024
 // workgroups in all (about one resident round: 4 per CU by LDS), with at
 // least 4 stages per split and at most 1 GiB of partial rows.
-static int pick_split(zv_context *c, int Mc, int precision)
+static int pick_split(zv_context *c, int Mc, int precision, int64_t tiles)
 {
-    int64_t tiles = 0;
-    for (int k = 0; k < 9; k++) tiles += c->grp_ntile[k];
     if (tiles <= 0) return 1;
     const bool single = precision == ZV_PREC_FP32 || precision == ZV_PREC_BF16;
     const int sub = single ? GroupSub<float>::SUB : GroupSub<double>::SUB;
     int64_t n = (1024 + tiles - 1) / tiles;
+    // a caller's choice (zv_set_tuning) replaces the count, within the
+    // same bounds of stages and partial-row memory
+    if (c->tune[ZV_TUNE_EST_SPLIT] > 0) n = c->tune[ZV_TUNE_EST_SPLIT];
     n = std::min<int64_t>(n, 64);
     n = std::min<int64_t>(n, std::max<int64_t>(1, Mc / sub / 4));
     n = std::min<int64_t>(n, 1 + ((int64_t)1 << 27) / std::max<int64_t>(1, c->nint));
@@ -975,6 +977,15 @@ static int group_ready(zv_context *c)
         g.insert(g.end(), upos[k].begin(), upos[k].end());
         g.insert(g.end(), usp[k].begin(), usp[k].end());
     }
+    c->h_pair_ucls.assign(npair, 0);
+    c->h_pair_uidx.assign(npair, 0);
+    for (int k = 0; k < 3; k++)
+        for (int u = 0; u < (int)upos[k].size(); u++)
+            for (int r = 0; r < kUnitRows[k]; r++) {
+                const int P = c->h_sp_comp[upos[k][u] + r];
+                c->h_pair_ucls[P] = k;
+                c->h_pair_uidx[P] = u;
+            }
     if (g.size() & 1) g.push_back(0); // int2 alignment of the tiles
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) {
@@ -1026,9 +1037,16 @@ static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const 
 // alone, so they run side by side on the main and the side streams (they
 // write disjoint quartets) and are joined back into the main stream.  With
 // nsplit > 1 every tile is also split over the samples of the chunk.
+// A subset of the tiles of every pair of unit classes (device lists), for
+// runs that refine only some quartets.
+struct TileSel {
+    const int2 *tiles[9];
+    int ntile[9];
+};
+
 template <typename T, bool BF>
 static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int Mround, int Mc,
-                               int nsplit, double *acc, double *mpart)
+                               int nsplit, double *acc, double *mpart, const TileSel *sel)
 {
     constexpr int SUB = GroupSub<T>::SUB;
     const int nstage = Mround / SUB;
@@ -1044,18 +1062,20 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
     // largest launches first
     int order[9];
     for (int k = 0; k < 9; k++) order[k] = k;
-    std::sort(order, order + 9, [&](int x, int y) { return c->grp_ntile[x] > c->grp_ntile[y]; });
+    auto count = [&](int k) { return sel ? sel->ntile[k] : c->grp_ntile[k]; };
+    std::sort(order, order + 9, [&](int x, int y) { return count(x) > count(y); });
     int nl = 0;
     for (int oi = 0; oi < 9; oi++) {
         const int a = order[oi] / 3, b = order[oi] % 3;
         {
-            const unsigned nt = (unsigned)c->grp_ntile[3 * a + b];
+            const unsigned nt = (unsigned)count(3 * a + b);
             if (!nt) continue;
             hipStream_t st = sts[nl++ % (1 + zv_context::NSIDE)];
             const int *urp = g + c->grp_unit_off[a], *urs = urp + c->grp_nunit[a];
             const int *ucp = g + c->grp_unit_off[b], *ucs = ucp + c->grp_nunit[b];
             const int nur = c->grp_nunit[a], nuc = c->grp_nunit[b];
-            const int2 *tiles = (const int2 *)g + c->grp_tile_off[3 * a + b];
+            const int2 *tiles =
+                sel ? sel->tiles[3 * a + b] : (const int2 *)g + c->grp_tile_off[3 * a + b];
 #define ZV_GROUP(NR, NC, TR, TC)                                                                  \
     launch_group_t<T, BF, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
                                       pos_pair, Mround, nsplit, mspan, Mc, acc, mpart)
@@ -1081,9 +1101,11 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
     return ZV_OK;
 }
 
-int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uint64_t stream,
-              int precision, const double *r1, const double *r2, const double *pi0, double *mean,
-              double *err, double *block_means)
+// nbl blocks of npts samples over all tiles (sel null) or a subset; the mean
+// is left in accS and the standard error in b_out2, both in triangle order.
+static int run_mc_core(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed,
+                       uint64_t stream, int precision, const double *r1, const double *r2,
+                       const double *pi0, double *block_means, const TileSel *sel)
 {
     const int64_t ntri = c->nint;
     const int np = c->npair, npp = c->npair_pad;
@@ -1103,7 +1125,9 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
     ZV_CHECK(zv_ensure(c, c->b_s1, sizeof(double) * ntri));
     ZV_CHECK(zv_ensure(c, c->b_s2, sizeof(double) * ntri));
     if (block_means) ZV_CHECK(zv_ensure(c, c->b_blk, sizeof(double) * ntri));
-    const int nsplit = precision == ZV_PREC_FP64_PARITY ? 1 : pick_split(c, Mc, precision);
+    int64_t ntiles = 0;
+    for (int k = 0; k < 9; k++) ntiles += sel ? sel->ntile[k] : c->grp_ntile[k];
+    const int nsplit = precision == ZV_PREC_FP64_PARITY ? 1 : pick_split(c, Mc, precision, ntiles);
     if (nsplit > 1) ZV_CHECK(zv_ensure(c, c->b_mpart, sizeof(double) * (nsplit - 1) * ntri));
     double *mpart = nsplit > 1 ? (double *)c->b_mpart.p : nullptr;
     double *accS = (double *)c->b_accS.p;
@@ -1169,10 +1193,10 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                     PhaseTimer tm(c, PH_EST);
                     if (precision == ZV_PREC_BF16)
                         ZV_CHECK((run_group_estimator<float, true>(c, comp, grp, Mround, Mc,
-                                                                   nsplit, accS, mpart)));
+                                                                   nsplit, accS, mpart, sel)));
                     else
                         ZV_CHECK((run_group_estimator<float, false>(c, comp, grp, Mround, Mc,
-                                                                    nsplit, accS, mpart)));
+                                                                    nsplit, accS, mpart, sel)));
                 } else {
                     double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
@@ -1184,7 +1208,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                     }
                     PhaseTimer tm(c, PH_EST);
                     ZV_CHECK((run_group_estimator<double, false>(c, comp, grp, Mround, Mc,
-                                                                 nsplit, accS, mpart)));
+                                                                 nsplit, accS, mpart, sel)));
                 }
             }
         }
@@ -1213,7 +1237,148 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
                            nbl, s1, s2, accS, (double *)c->b_out2.p);
         ZV_HIPCHECK(c, hipGetLastError());
     }
-    ZV_CHECK(zv_keep_result(c, accS, ZV_RESULT_MEAN, mean));
+    return ZV_OK;
+}
+
+int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uint64_t stream,
+              int precision, const double *r1, const double *r2, const double *pi0, double *mean,
+              double *err, double *block_means)
+{
+    ZV_CHECK(run_mc_core(c, nbl, npts, mode, seed, stream, precision, r1, r2, pi0, block_means,
+                         nullptr));
+    ZV_CHECK(zv_keep_result(c, (const double *)c->b_accS.p, ZV_RESULT_MEAN, mean));
     ZV_CHECK(zv_keep_result(c, (const double *)c->b_out2.p, ZV_RESULT_STDERR, err));
     return ZV_OK;
+}
+
+// ---- targeted refinement ------------------------------------------------------
+namespace {
+// kcp positions with err > target: count, then (after the list is sized) fill
+__global__ void k_select(int64_t n, const double *__restrict__ err, double target,
+                         unsigned long long *count, int64_t *list, int64_t cap)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || !(err[t] > target)) return;
+    const unsigned long long at = atomicAdd(count, 1ull);
+    if (list && (int64_t)at < cap) list[at] = t;
+}
+
+// reference position kcp -> electron-1 pair (is, ks), electron-2 pair (js, ls)
+// and the triangle slot; the (l, k, j) run holding kcp is the last base-table
+// entry not above it (src/build_mapping_ijkl.F90:16-43 order)
+__global__ void k_kcp_pairs(int64_t nsel, const int64_t *__restrict__ list,
+                            const int64_t *__restrict__ kbase, int n, int *pr, int *pc,
+                            int64_t *tri)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsel) return;
+    const int64_t kcp = list[x];
+    int64_t lo = 0, hi = (int64_t)n * n * n; // first entry above kcp
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (kbase[mid] <= kcp) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t e = lo - 1;
+    const int j = (int)(e % n) + 1, k = (int)((e / n) % n) + 1, l = (int)(e / ((int64_t)n * n)) + 1;
+    const int start = (k > j ? k : j) + ((l > k && j >= k) ? 1 : 0);
+    const int i = start + (int)(kcp - kbase[e]);
+    const int P = zv_pidx(i, k), Q = zv_pidx(j, l);
+    pr[x] = P > Q ? P : Q;
+    pc[x] = P > Q ? Q : P;
+    tri[x] = P > Q ? zv_tri(P, Q) : zv_tri(Q, P);
+}
+
+// kept results of the selected quartets <- this run's (triangle order)
+__global__ void k_merge_sel(int64_t nsel, const int64_t *__restrict__ list,
+                            const int64_t *__restrict__ tri, const double *__restrict__ mean_t,
+                            const double *__restrict__ err_t, double *res_mean, double *res_err)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsel) return;
+    res_mean[list[x]] = mean_t[tri[x]];
+    res_err[list[x]] = err_t[tri[x]];
+}
+} // namespace
+
+int zv_run_mc_refine(zv_context *c, int nbl, int64_t npts, uint64_t seed, uint64_t stream,
+                     int precision, double target, int64_t *n_refined, double *max_err)
+{
+    if (!c->res_valid[ZV_RESULT_MEAN] || !c->res_valid[ZV_RESULT_STDERR])
+        return zv_fail(c, ZV_ERR_STATE,
+                       "zv_mc_refine needs the device results of a zv_mc_blocks run");
+    ZV_CHECK(group_ready(c));
+    const int64_t ntri = c->nint;
+    double *res_mean = (double *)c->b_res[ZV_RESULT_MEAN].p;
+    double *res_err = (double *)c->b_res[ZV_RESULT_STDERR].p;
+    // 1. the quartets above the target
+    ZV_CHECK(zv_ensure(c, c->b_misc, 64));
+    unsigned long long *d_count = (unsigned long long *)c->b_misc.p;
+    ZV_HIPCHECK(c, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_select, dim3(nblk(ntri, 256)), dim3(256), 0, c->stream, ntri,
+                       (const double *)res_err, target, d_count, (int64_t *)nullptr, (int64_t)0);
+    ZV_HIPCHECK(c, hipGetLastError());
+    unsigned long long hcount = 0;
+    ZV_HIPCHECK(c, hipMemcpyAsync(&hcount, d_count, sizeof(hcount), hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t nsel = (int64_t)hcount;
+    if (n_refined) *n_refined = nsel;
+    if (nsel == 0) return zv_run_result_max(c, ZV_RESULT_STDERR, max_err);
+    // list (int64), pairs (2 int), triangle slots (int64) in b_list
+    const size_t lbytes = (size_t)nsel * (8 + 8 + 8);
+    ZV_CHECK(zv_ensure(c, c->b_list, lbytes));
+    int64_t *d_list = (int64_t *)c->b_list.p, *d_tri = d_list + nsel;
+    int *d_pr = (int *)(d_tri + nsel), *d_pc = d_pr + nsel;
+    ZV_HIPCHECK(c, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_select, dim3(nblk(ntri, 256)), dim3(256), 0, c->stream, ntri,
+                       (const double *)res_err, target, d_count, d_list, nsel);
+    hipLaunchKernelGGL(k_kcp_pairs, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel,
+                       (const int64_t *)d_list, (const int64_t *)c->d_kbase, c->n, d_pr, d_pc,
+                       d_tri);
+    ZV_HIPCHECK(c, hipGetLastError());
+    // 2. the tiles holding them, per pair of unit classes
+    std::vector<int> hpr(nsel), hpc(nsel);
+    ZV_HIPCHECK(c, hipMemcpyAsync(hpr.data(), d_pr, sizeof(int) * nsel, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(hpc.data(), d_pc, sizeof(int) * nsel, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int64_t> keys[9];
+    for (int64_t x = 0; x < nsel; x++) {
+        const int a = c->h_pair_ucls[hpr[x]], b = c->h_pair_ucls[hpc[x]];
+        const int RU = 16 * kUnitsPerThread[a][b][0], CU = 16 * kUnitsPerThread[a][b][1];
+        const int64_t rt = c->h_pair_uidx[hpr[x]] / RU, ct = c->h_pair_uidx[hpc[x]] / CU;
+        keys[3 * a + b].push_back((rt << 32) | ct);
+    }
+    std::vector<int> htiles;
+    int64_t toff[9];
+    TileSel sel;
+    for (int k = 0; k < 9; k++) {
+        std::sort(keys[k].begin(), keys[k].end());
+        keys[k].erase(std::unique(keys[k].begin(), keys[k].end()), keys[k].end());
+        toff[k] = (int64_t)htiles.size() / 2;
+        for (int64_t key : keys[k]) {
+            htiles.push_back((int)(key >> 32));
+            htiles.push_back((int)(key & 0xffffffff));
+        }
+        sel.ntile[k] = (int)keys[k].size();
+    }
+    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(int) * std::max<size_t>(htiles.size(), 2)));
+    ZV_HIPCHECK(c, hipMemcpyAsync(c->b_part.p, htiles.data(), sizeof(int) * htiles.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    for (int k = 0; k < 9; k++) sel.tiles[k] = (const int2 *)c->b_part.p + toff[k];
+    // 3. a run over those tiles, merged into the kept results for the
+    // selected quartets only
+    ZV_CHECK(run_mc_core(c, nbl, npts, ZV_SAMPLES_PHILOX, seed, stream, precision, nullptr,
+                         nullptr, nullptr, nullptr, &sel));
+    {
+        PhaseTimer tm(c, PH_COPY);
+        hipLaunchKernelGGL(k_merge_sel, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel,
+                           (const int64_t *)d_list, (const int64_t *)d_tri,
+                           (const double *)c->b_accS.p, (const double *)c->b_out2.p, res_mean,
+                           res_err);
+        ZV_HIPCHECK(c, hipGetLastError());
+    }
+    return zv_run_result_max(c, ZV_RESULT_STDERR, max_err);
 }

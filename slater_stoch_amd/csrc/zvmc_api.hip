@@ -275,7 +275,7 @@ extern "C" int zv_destroy(zv_context *c)
                       &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc,
                       &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
                       &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist,
-                      &c->b_gmeta, &c->b_res[0], &c->b_res[1], &c->b_res[2]};
+                      &c->b_gmeta, &c->b_res[0], &c->b_res[1], &c->b_res[2], &c->b_mpart};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -299,6 +299,17 @@ extern "C" int zv_set_verbose(zv_context *c, int level)
 {
     if (!c) return ZV_ERR_ARG;
     c->verbose = level;
+    return ZV_OK;
+}
+
+extern "C" int zv_set_tuning(zv_context *c, int key, int64_t value)
+{
+    ZV_SERIALIZE;
+    if (!c) return ZV_ERR_ARG;
+    if (key < 0 || key > ZV_TUNE_ERI_STREAMS)
+        return zv_fail(c, ZV_ERR_ARG, "unknown tuning key %d", key);
+    if (value < 0) return zv_fail(c, ZV_ERR_ARG, "tuning value %lld < 0", (long long)value);
+    c->tune[key] = value;
     return ZV_OK;
 }
 
@@ -335,6 +346,11 @@ static int upload(zv_context *c, T **dst, const T *src, size_t count)
 
 static int hermite_count(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
 
+static int set_basis_tables(zv_context *c, int n, const double *center, const int *npower,
+                            const int *n_sto, const int *i_type, const double *zeta,
+                            const int *n_gauss, int ldg, const double *c_gauss,
+                            const double *g_gauss, double a_zv);
+
 extern "C" int zv_set_basis(zv_context *c, int n, const double *center, const int *npower,
                             const int *n_sto, const int *i_type, const double *zeta,
                             const int *n_gauss, int ldg, const double *c_gauss,
@@ -362,10 +378,38 @@ extern "C" int zv_set_basis(zv_context *c, int n, const double *center, const in
         if (n_sto[i] < 0 || n_sto[i] > 4)
             return zv_fail(c, ZV_ERR_ARG, "n_STO(%d)=%d outside 0..4", i + 1, n_sto[i]);
     }
+    // the largest pair order is that of the AO with the largest order paired
+    // with itself; checked before the old basis is released
+    int lmax = 0, imax = 0;
+    for (int i = 0; i < n; i++) {
+        const int l = npower[3 * i] + npower[3 * i + 1] + npower[3 * i + 2] +
+                      (i_type[i] == 2 ? 2 : 0);
+        if (l > lmax) lmax = l, imax = i;
+    }
+    if (2 * lmax > ZV_LPAIR_MAX)
+        return zv_fail(c, ZV_ERR_ARG, "pair (%d,%d) angular momentum %d > %d", imax + 1, imax + 1,
+                       2 * lmax, ZV_LPAIR_MAX);
     ZV_HIPCHECK(c, hipSetDevice(c->device));
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     free_basis(c);
+    // from here on a failure leaves no basis: later calls get ZV_ERR_STATE
+    // instead of running on a half-built one
+    const int rc = set_basis_tables(c, n, center, npower, n_sto, i_type, zeta, n_gauss, ldg,
+                                    c_gauss, g_gauss, a_zv);
+    if (rc != ZV_OK) {
+        free_basis(c);
+        c->n = 0;
+        c->npair = c->npair_pad = 0;
+        c->nint = 0;
+    }
+    return rc;
+}
 
+static int set_basis_tables(zv_context *c, int n, const double *center, const int *npower,
+                            const int *n_sto, const int *i_type, const double *zeta,
+                            const int *n_gauss, int ldg, const double *c_gauss,
+                            const double *g_gauss, double a_zv)
+{
     c->n = n;
     c->a_zv = a_zv;
     c->npair = n * (n + 1) / 2;
@@ -646,6 +690,51 @@ extern "C" int zv_davidson_clean(zv_context *c, double *eri, int *n_removed)
     ZV_NEED_BASIS(c);
     if (!eri) return zv_fail(c, ZV_ERR_ARG, "null eri");
     int rc = zv_run_davidson_clean(c, eri, n_removed);
+    if (rc == ZV_OK) rc = zv_collect_timings(c);
+    return rc;
+}
+
+extern "C" int zv_mc_refine(zv_context *c, int nbl, int64_t npts, uint64_t seed,
+                            uint64_t stream, int precision, double target, int64_t *n_refined,
+                            double *max_err)
+{
+    ZV_SERIALIZE;
+    ZV_NEED_BASIS(c);
+    if (nbl < 2) return zv_fail(c, ZV_ERR_ARG, "nbl must be >= 2");
+    if (npts < ZV_NW || npts % ZV_NW != 0)
+        return zv_fail(c, ZV_ERR_ARG, "npts_two_elec must be a multiple of nw");
+    if (precision != ZV_PREC_FP64 && precision != ZV_PREC_FP32 && precision != ZV_PREC_BF16)
+        return zv_fail(c, ZV_ERR_ARG, "zv_mc_refine runs the group estimator (FP64, FP32, BF16)");
+    if (!(target > 0.0)) return zv_fail(c, ZV_ERR_ARG, "target must be positive");
+    int rc = zv_run_mc_refine(c, nbl, npts, seed, stream, precision, target, n_refined, max_err);
+    if (rc == ZV_OK) rc = zv_collect_timings(c);
+    return rc;
+}
+
+extern "C" int zv_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int sketch,
+                             double *eri, double *ritz, int *n_removed)
+{
+    ZV_SERIALIZE;
+    ZV_NEED_BASIS(c);
+    if (!eri) return zv_fail(c, ZV_ERR_ARG, "null eri");
+    if (rank < 1 || q < 0) return zv_fail(c, ZV_ERR_ARG, "rank >= 1 and q >= 0 required");
+    if (sketch != ZV_SKETCH_FP64 && sketch != ZV_SKETCH_FP32)
+        return zv_fail(c, ZV_ERR_ARG, "unknown sketch precision %d", sketch);
+    int rc = zv_run_rsvd_clean(c, rank, q, seed, sketch == ZV_SKETCH_FP32, eri, ritz, n_removed);
+    if (rc == ZV_OK) rc = zv_collect_timings(c);
+    return rc;
+}
+
+extern "C" int zv_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
+                                  double *evals, double *resid, int *n_removed, int *iters)
+{
+    ZV_SERIALIZE;
+    ZV_NEED_BASIS(c);
+    if (!eri) return zv_fail(c, ZV_ERR_ARG, "null eri");
+    if (nev < 1 || nev > 64) return zv_fail(c, ZV_ERR_ARG, "nev %d outside 1..64", nev);
+    if (!(tol > 0.0) || max_iter < 1)
+        return zv_fail(c, ZV_ERR_ARG, "tol > 0 and max_iter >= 1 required");
+    int rc = zv_run_davidson_lowest(c, nev, tol, max_iter, eri, evals, resid, n_removed, iters);
     if (rc == ZV_OK) rc = zv_collect_timings(c);
     return rc;
 }

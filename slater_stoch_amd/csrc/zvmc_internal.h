@@ -32,6 +32,7 @@ enum Phase { PH_SAMPLE = 0, PH_DENS, PH_EST, PH_STATS, PH_PRIM, PH_ERI, PH_COPY,
 struct zv_context {
     int device = 0;
     int verbose = 0;
+    int64_t tune[ZV_TUNE_N] = {};      // zv_set_tuning values (0 = automatic)
     hipStream_t stream = nullptr;
     // extra streams (default priority) for independent launches that are
     // each too small to fill the device; forked from and joined back into
@@ -107,6 +108,7 @@ struct zv_context {
     int64_t grp_possp_off = 0;
     int grp_nsp_pad = 0;
     std::vector<int> h_gmeta;
+    std::vector<int> h_pair_ucls, h_pair_uidx; // unit class and index of each AO pair
     // planned shell work of the two Schwarz phases (host copies kept alive for the copies)
     DevBuf b_sitem1, b_sgroup1, b_part1, b_sitem2, b_sgroup2, b_part2, b_plist;
     std::vector<char> h_plan1, h_plan2;
@@ -186,6 +188,12 @@ int zv_keep_result(zv_context *c, const double *d_tri, int which, double *host_o
 int zv_run_result_max(zv_context *c, int which, double *value);
 int zv_run_svd_clean(zv_context *c, int rank, int q, uint64_t seed, double *eri, int *n_removed);
 int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed);
+int zv_run_mc_refine(zv_context *c, int nbl, int64_t npts, uint64_t seed, uint64_t stream,
+                     int precision, double target, int64_t *n_refined, double *max_err);
+int zv_run_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int sketch32, double *eri,
+                      double *ritz, int *n_removed);
+int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
+                           double *evals, double *resid, int *n_removed, int *iters);
 
 #define ZV_CHECK(call)                                                                            \
     do {                                                                                          \

@@ -131,6 +131,12 @@ module zvmc
       type(c_ptr), value :: ctx
       integer(c_int), value :: level
     end function
+    integer(c_int) function zv_set_tuning(ctx, key, val) bind(C, name="zv_set_tuning")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: key
+      integer(c_int64_t), value :: val
+    end function
   end interface
 
 contains

@@ -50,7 +50,13 @@ SIGNATURES = {
     "zv_last_eri_stats": (I32, [VP, VP, VP, VP, VP]),
     "zv_get_result": (I32, [VP, I32, VP]),
     "zv_result_max": (I32, [VP, I32, VP]),
+    "zv_set_tuning": (I32, [VP, I32, I64]),
+    "zv_mc_refine": (I32, [VP, I32, I64, U64, U64, I32, F64, VP, VP]),
+    "zv_rsvd_clean": (I32, [VP, I32, I32, U64, I32, VP, VP, VP]),
+    "zv_davidson_lowest": (I32, [VP, I32, F64, I32, VP, VP, VP, VP, VP]),
 }
+
+TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS, TUNE_ERI_STREAMS = 0, 1, 2, 3
 
 RESULT_ERI, RESULT_MEAN, RESULT_STDERR = 0, 1, 2
 
@@ -133,6 +139,12 @@ class Engine:
             self.lib.zv_last_error(self.h, buf, 512)
             raise ZVError(rc, buf.value.decode(errors="replace"))
 
+    def set_tuning(self, key, value):
+        """Tuning knob (TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS,
+        TUNE_ERI_STREAMS); 0
+        restores the library's automatic choice."""
+        self._check(self.lib.zv_set_tuning(self.h, int(key), int(value)))
+
     # ------------------------------------------------------------ basis
     def set_basis(self, bas):
         self.basis = bas
@@ -160,9 +172,10 @@ class Engine:
         (src/two_e_slater.F90:318-365); returns the number of one-centre
         quartets."""
         nint = self.basis.nint
-        assert ijkl.dtype == np.float64 and ijkl.flags.c_contiguous and ijkl.size == nint
-        if err is not None:
-            assert err.dtype == np.float64 and err.flags.c_contiguous and err.size == nint
+        for name, a in (("ijkl", ijkl), ("err", err)):
+            if a is not None and not (a.dtype == np.float64 and a.flags.c_contiguous
+                                      and a.size == nint):
+                raise ValueError(f"{name} must be a contiguous float64 array of {nint} entries")
         nm = np.zeros(1, np.int64)
         self._check(self.lib.zv_exact_monocenter(self.h, _p(ijkl), _p(err), _p(nm)))
         return int(nm[0])
@@ -218,7 +231,11 @@ class Engine:
         blk = np.zeros((nbl, nint)) if block_means else None
         if samples is not None:
             r1, r2, pi0 = [np.ascontiguousarray(a, np.float64) for a in samples]
-            assert r1.shape == (nbl * npts, 3) and pi0.shape == (nbl * npts,)
+            m = nbl * npts
+            for name, a, shape in (("r1", r1, (m, 3)), ("r2", r2, (m, 3)), ("pi0", pi0, (m,))):
+                if a.shape != shape:
+                    raise ValueError(f"{name} has shape {a.shape}, expected {shape} "
+                                     f"(nbl * npts = {m} samples)")
             mode = SAMPLES_FIXED
         else:
             r1 = r2 = pi0 = None
@@ -226,6 +243,17 @@ class Engine:
         self._check(self.lib.zv_mc_blocks(self.h, nbl, npts, mode, seed, stream, prec, _p(r1),
                                           _p(r2), _p(pi0), _p(mean), _p(err), _p(blk)))
         return (mean, err, blk) if block_means else (mean, err)
+
+    def mc_refine(self, nbl, npts, precision="fp32", target=1e-6, seed=0, stream=1):
+        """Refine the device-resident results: quartets with stderr > target
+        get a fresh run of nbl x npts samples over the tiles holding them.
+        Returns (quartets refined, largest standard error afterwards)."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        nref = np.zeros(1, np.int64)
+        emax = np.zeros(1)
+        self._check(self.lib.zv_mc_refine(self.h, int(nbl), int(npts), int(seed), int(stream),
+                                          prec, float(target), _p(nref), _p(emax)))
+        return int(nref[0]), float(emax[0])
 
     def draw_samples(self, m, seed=0, stream=0, m0=0):
         r1, r2, pi0 = np.zeros((m, 3)), np.zeros((m, 3)), np.zeros(m)
@@ -258,6 +286,30 @@ class Engine:
         nrem = np.zeros(1, np.int32)
         self._check(self.lib.zv_davidson_clean(self.h, _p(e), _p(nrem)))
         return e, int(nrem[0])
+
+    def rsvd_clean(self, eri, rank, q=2, seed=0, sketch="fp32"):
+        """Randomized Rayleigh-Ritz cleanup of the pair supermatrix: (cleaned
+        eri, Ritz values ascending, number of negative pairs removed)."""
+        e = np.array(eri, np.float64, copy=True)
+        r = min(int(rank), self.basis.npair)
+        ritz = np.zeros(r)
+        nrem = np.zeros(1, np.int32)
+        self._check(self.lib.zv_rsvd_clean(self.h, int(rank), int(q), int(seed),
+                                           {"fp64": 0, "fp32": 1}[sketch], _p(e), _p(ritz),
+                                           _p(nrem)))
+        return e, ritz, int(nrem[0])
+
+    def davidson_lowest(self, eri, nev=10, tol=1e-9, max_iter=1000):
+        """Block Davidson on the pair supermatrix: (cleaned eri, the nev
+        lowest Ritz values, their residual norms, number removed,
+        iterations)."""
+        e = np.array(eri, np.float64, copy=True)
+        ev, res = np.zeros(int(nev)), np.zeros(int(nev))
+        nrem = np.zeros(1, np.int32)
+        its = np.zeros(1, np.int32)
+        self._check(self.lib.zv_davidson_lowest(self.h, int(nev), float(tol), int(max_iter),
+                                                _p(e), _p(ev), _p(res), _p(nrem), _p(its)))
+        return e, ev, res, int(nrem[0]), int(its[0])
 
     def eri_stats(self):
         """Per ERI kernel of the last gauss_eri / gauss_ijkl: {key: dict(ms,

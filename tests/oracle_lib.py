@@ -160,3 +160,46 @@ class Oracle:
 
 def golden(name):
     return np.load(ROOT / "tests" / "golden" / name, allow_pickle=False)
+
+
+REF_LIB = ROOT / "oracle" / "_ref" / "libref.so"
+
+
+class Reference:
+    """The compiled reference (oracle/_ref/libref.so: the reference's own
+    Fortran sources for the hot path plus the bind(C) harness
+    oracle/ref_harness.F90), for timing its CPU path.  The deck is written in
+    the reference's own formats and read by its own routines.  Raises
+    FileNotFoundError when the library has not been built."""
+
+    def __init__(self, bas, ng0=20):
+        import tempfile
+        from slater_stoch_amd import configs
+        if not REF_LIB.exists():
+            raise FileNotFoundError(REF_LIB)
+        lib = ctypes.CDLL(str(REF_LIB))
+        lib.ref_setup.restype = ctypes.c_int
+        lib.ref_setup.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        lib.ref_gauss_ijkl.restype = ctypes.c_double
+        lib.ref_gauss_ijkl.argtypes = [ctypes.c_int] * 4
+        lib.ref_mc_fixed.argtypes = [ctypes.c_int, ctypes.c_int, VP, VP, VP,
+                                     ctypes.c_int64] + [VP] * 6
+        self.lib = lib
+        d = Path(tempfile.mkdtemp())
+        configs.write_deck(bas, d, name="mol", ng0=ng0)
+        n = lib.ref_setup(str(d / "mol").encode(), str(d / "mol.basis").encode(), ng0, 0, 0,
+                          100.0, 0, bas.a_zv)
+        if n != bas.n:
+            raise RuntimeError(f"reference read {n} orbitals, expected {bas.n}")
+
+    def gauss_ijkl(self, i, k, j, l):
+        return self.lib.ref_gauss_ijkl(int(i), int(k), int(j), int(l))
+
+    def mc_fixed(self, nbl, npts, r1, r2, pi0, quartets):
+        is_, js_, ks_, ls_ = [np.ascontiguousarray(a, np.int32) for a in quartets]
+        mean, err = np.zeros(is_.size), np.zeros(is_.size)
+        self.lib.ref_mc_fixed(nbl, npts, P(np.ascontiguousarray(r1)),
+                              P(np.ascontiguousarray(r2)), P(np.ascontiguousarray(pi0)),
+                              is_.size, P(is_), P(js_), P(ks_), P(ls_), P(mean), P(err))
+        return mean, err

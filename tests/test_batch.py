@@ -29,21 +29,34 @@ def test_next_npts_rounds_scales_and_caps():
 
 
 class FakeEngine:
-    """err = s0 / sqrt(nbl * npts) for every quartet; records the calls."""
+    """err = s0 / sqrt(nbl * npts) for every quartet of a run (for the
+    quartets above the target in a refinement); records the calls."""
 
     def __init__(self, s0, nint=5):
         self.s0, self.nint, self.calls = s0, nint, []
+        self.mean, self.err = np.zeros(nint), np.zeros(nint)
 
     def set_basis(self, bas):
         self.bas = bas
 
-    def gauss_eri(self):
-        return np.zeros(self.nint), None, 0
+    def gauss_eri(self, device_only=False):
+        return (None if device_only else np.zeros(self.nint)), None, 0
 
-    def mc_blocks(self, nbl, npts, precision, seed=0, stream=0):
+    def mc_blocks(self, nbl, npts, precision, seed=0, stream=0, device_only=False):
         self.calls.append((nbl, npts, precision, seed, stream))
-        err = np.full(self.nint, self.s0 / np.sqrt(nbl * npts))
-        return np.full(self.nint, float(stream)), err
+        self.err = np.full(self.nint, self.s0 / np.sqrt(nbl * npts))
+        self.mean = np.full(self.nint, float(stream))
+        return (None, None) if device_only else (self.mean.copy(), self.err.copy())
+
+    def mc_refine(self, nbl, npts, precision, target=1e-6, seed=0, stream=1):
+        self.calls.append((nbl, npts, precision, seed, stream))
+        sel = self.err > target
+        self.err[sel] = self.s0 / np.sqrt(nbl * npts)
+        self.mean[sel] = float(stream)
+        return int(sel.sum()), float(self.err.max())
+
+    def result_max(self, which):
+        return float(self.err.max())
 
 
 class FakeBasis:

@@ -78,3 +78,45 @@ def test_analytic_eris_repeat_bit_for_bit(engine):
     first = engine.get_result(Z.RESULT_ERI)
     engine.gauss_eri(device_only=True)
     np.testing.assert_array_equal(engine.get_result(Z.RESULT_ERI), first)
+
+
+def _with_pair_order_too_high(bas):
+    """A copy of the basis whose first AO is an i_type 2 AO with powers
+    (2, 2, 1): every axis passes the per-axis check, but its pair with
+    itself has order 2 (5 + 2) = 14, above the supported 8."""
+    import copy
+    b = copy.deepcopy(bas)
+    b.npower = np.array(b.npower, copy=True)
+    b.i_type = np.array(b.i_type, copy=True)
+    b.npower[0] = (2, 2, 1)
+    b.i_type[0] = 2
+    return b
+
+
+def test_rejected_basis_keeps_the_previous_one(engine):
+    """A basis rejected by validation is refused before anything is freed:
+    the previous basis and its kept results stay usable.  No kernel runs on
+    the rejected input."""
+    from slater_stoch_amd import zvmc as Z
+    good = configs.water_minimal(ng0=4)
+    engine.set_basis(good)
+    eri, _, _ = engine.gauss_eri()
+    with pytest.raises(Z.ZVError, match="angular momentum 14 > 8"):
+        engine.set_basis(_with_pair_order_too_high(good))
+    engine.basis = good
+    np.testing.assert_array_equal(engine.get_result(Z.RESULT_ERI), eri)
+    again, _, _ = engine.gauss_eri()
+    np.testing.assert_array_equal(again, eri)
+
+
+def test_rejected_basis_on_a_fresh_context_leaves_no_basis():
+    from slater_stoch_amd import zvmc as Z
+    good = configs.water_minimal(ng0=4)
+    with Z.Engine(0) as eng:
+        with pytest.raises(Z.ZVError, match="angular momentum"):
+            eng.set_basis(_with_pair_order_too_high(good))
+        eng.basis = good
+        with pytest.raises(Z.ZVError, match="zv_set_basis has not been called"):
+            eng.get_result(Z.RESULT_ERI)
+        with pytest.raises(Z.ZVError, match="zv_set_basis has not been called"):
+            eng.gauss_eri()
