@@ -117,6 +117,22 @@ int zv_one_e(zv_context *ctx, int nnuc, const double *charge, const double *nuc,
 int zv_gauss_ijkl_list(zv_context *ctx, int64_t nq, const int *qi, const int *qk,
                        const int *qj, const int *ql, double *out);
 
+/* zv_gauss_ijkl_list without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and values (bit for bit) as zv_gauss_ijkl_list;
+ * the device work runs in a fixed static device array of the library
+ * (2 MiB per device that loads it, held until the process ends), so
+ * nq <= ZV_NOALLOC_MAXQ (ZV_ERR_ARG above).  The primitive-pair tables of
+ * the basis must exist: the first zv_gauss_eri or zv_gauss_ijkl_list after
+ * zv_set_basis builds them (ZV_ERR_STATE otherwise).  Every call on a
+ * device uses that device's one array: calls run one at a time under the
+ * library's call lock, and callers that bypass it (several processes on one
+ * GPU have separate arrays) must not overlap calls on one device. */
+#define ZV_NOALLOC_MAXQ 65536
+int zv_gauss_ijkl_list_noalloc(zv_context *ctx, int64_t nq, const int *qi, const int *qk,
+                               const int *qj, const int *ql, double *out);
+
 /* ZVMC correction blocks, src/two_e_slater.F90:386-496: nbl blocks of npts
  * samples (npts a multiple of 40, src/two_e_slater.F90:148).  mode
  * ZV_SAMPLES_FIXED reads r1(3,nbl*npts), r2(3,nbl*npts), pi0(nbl*npts);

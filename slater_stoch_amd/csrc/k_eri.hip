@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 
 #include "eri_device.h"
 #include "zvmc_internal.h"
@@ -365,6 +366,32 @@ bool launch_class_t(int LB, int LK, unsigned grid, hipStream_t st, PP pp, QSrc S
 } // namespace
 
 // ====================================================================== host
+// Device pointers of the primitive-pair tables inside b_pp, b_ppE, b_ppcnt.
+static PP pp_view(zv_context *c)
+{
+    PP pp;
+    const int np = c->npair;
+    double *base = (double *)c->b_pp.p;
+    size_t T = (size_t)c->pp_total;
+    pp.p = base;
+    pp.ip = base + T;
+    pp.Px = base + 2 * T;
+    pp.Py = base + 3 * T;
+    pp.Pz = base + 4 * T;
+    pp.c = base + 5 * T;
+    pp.E = (double *)c->b_ppE.p;
+    char *m = (char *)c->b_ppcnt.p;
+    int64_t *d_off = (int64_t *)m;
+    int64_t *d_eoff = d_off + np;
+    int *d_cnt = (int *)(d_eoff + np);
+    pp.off = d_off;
+    pp.eoff = d_eoff;
+    pp.cnt = d_cnt;
+    pp.cap = d_cnt + np;
+    pp.L = d_cnt + 2 * np;
+    return pp;
+}
+
 static int ensure_primpairs(zv_context *c, PP &pp)
 {
     int np = c->npair;
@@ -435,23 +462,21 @@ static double component_flops(int LB, int LK)
 // Launch one class over an explicit quartet list S, splitting bra primitive
 // pairs when there are too few quartets to fill the device.  pq and flops
 // (primitive quartets and algorithmic flops of the list) feed the statistics.
-static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, double *out,
-                        double pq, double flops)
+// Split of a class launch over the bra primitive pairs: short lists are
+// spread over up to 32 waves per quartet, about 16384 waves in all.  The
+// partial sums then take nq * nsplit < 16384 + nq doubles.
+static int class_split(int64_t nq)
 {
-    if (S.nq <= 0) return ZV_OK;
     const int64_t target_waves = 16384;
-    int nsplit = 1;
-    if (S.nq < target_waves) nsplit = (int)std::min<int64_t>(32, (target_waves + S.nq - 1) / S.nq);
-    double *part = nullptr;
-    if (nsplit > 1) {
-        ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * S.nq * nsplit));
-        part = (double *)c->b_part.p;
-    }
+    if (nq >= target_waves) return 1;
+    return (int)std::min<int64_t>(32, (target_waves + nq - 1) / nq);
+}
+
+// The kernels of one class: no allocation, no timing.
+static int launch_class_raw(zv_context *c, const PP &pp, QSrc S, int LB, int LK, double *out,
+                            double *part, int nsplit)
+{
     int64_t items = S.nq * nsplit;
-    const int cls = LB * 6 + LK;
-    c->cls_pq[cls] += pq;
-    c->cls_flops[cls] += flops;
-    PhaseTimer tm(c, PH_ERI, cls);
     if (LB <= ZV_LMAX_T && LK <= ZV_LMAX_T) {
         if (!launch_class_t(LB, LK, nblk(items * 64, 256), c->stream, pp, S, nsplit, out, part))
             return zv_fail(c, ZV_ERR_STATE, "no ERI kernel for class (%d,%d)", LB, LK);
@@ -466,6 +491,23 @@ static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, dou
         ZV_HIPCHECK(c, hipGetLastError());
     }
     return ZV_OK;
+}
+
+static int launch_class(zv_context *c, const PP &pp, QSrc S, int LB, int LK, double *out,
+                        double pq, double flops)
+{
+    if (S.nq <= 0) return ZV_OK;
+    const int nsplit = class_split(S.nq);
+    double *part = nullptr;
+    if (nsplit > 1) {
+        ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * S.nq * nsplit));
+        part = (double *)c->b_part.p;
+    }
+    const int cls = LB * 6 + LK;
+    c->cls_pq[cls] += pq;
+    c->cls_flops[cls] += flops;
+    PhaseTimer tm(c, PH_ERI, cls);
+    return launch_class_raw(c, pp, S, LB, LK, out, part, nsplit);
 }
 
 // Component quartets (bra, ket) of pairs beyond the templated orders, written
@@ -664,5 +706,122 @@ int zv_run_gauss_list(zv_context *c, int64_t nq, const int *qi, const int *qk, c
                                   c->stream));
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     for (int64_t t = 0; t < nq; t++) out[perm[t]] = h[t];
+    return ZV_OK;
+}
+
+// ---- allocation-free list mode (zv_gauss_ijkl_list_noalloc) ---------------
+// The device memory of the allocation-free entry point: a file-scope array,
+// one copy per device that loads the library, held until the process ends.
+// Layout for up to kNaQ quartets: bra pair, ket pair and list position of
+// each quartet in class order (int), the values in class order and in list
+// order (double), then the split partial sums of one class launch (fewer
+// than kNaPart doubles, see class_split).  Every entry that is read is
+// written earlier in the same call, so nothing needs clearing.
+constexpr size_t kNaQ = ZV_NOALLOC_MAXQ;
+constexpr size_t kNaPart = 32768;
+constexpr size_t kNaBytes = 3 * kNaQ * sizeof(int) + (2 * kNaQ + kNaPart) * sizeof(double);
+static_assert((3 * kNaQ * sizeof(int)) % sizeof(double) == 0, "double alignment");
+static __device__ __align__(256) char scratch[kNaBytes];
+
+// Device address of `scratch` per device, from hipGetSymbolAddress at the
+// first context on that device (guarded by the library's call lock).  On the
+// host the array's name is not a device address; only these are used.
+static std::map<int, char *> g_scratch_ptr;
+
+int zv_noalloc_setup(zv_context *c)
+{
+    if (g_scratch_ptr.count(c->device)) return ZV_OK;
+    char *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return zv_hip(c, e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return zv_fail(c, ZV_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[c->device] = ptr;
+    return ZV_OK;
+}
+
+// list position -> class order, written back in list order
+__global__ void k_unpermute(int64_t nq, const int *__restrict__ pos,
+                            const double *__restrict__ in, double *__restrict__ out)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < nq) out[pos[x]] = in[x];
+}
+
+// zv_run_gauss_list without allocation: the same buckets, kernels and
+// summation order (so the same values, bit for bit), with the device work
+// in `scratch` and the host bookkeeping in static arrays (calls run one at
+// a time under the library's call lock).  Needs the primitive-pair tables
+// of the current basis, which the first zv_gauss_eri or zv_gauss_ijkl_list
+// after zv_set_basis builds.
+int zv_run_gauss_list_noalloc(zv_context *c, int64_t nq, const int *qi, const int *qk,
+                              const int *qj, const int *ql, double *out)
+{
+    auto sp = g_scratch_ptr.find(c->device);
+    if (sp == g_scratch_ptr.end())
+        return zv_fail(c, ZV_ERR_STATE, "allocation-free scratch is not set up on device %d",
+                       c->device);
+    if (nq < 0 || nq > (int64_t)kNaQ)
+        return zv_fail(c, ZV_ERR_ARG, "%lld quartets: the allocation-free list holds at most %lld",
+                       (long long)nq, (long long)kNaQ);
+    if (!c->pp_ready)
+        return zv_fail(c, ZV_ERR_STATE,
+                       "the primitive-pair tables of this basis are not built: call zv_gauss_eri "
+                       "or zv_gauss_ijkl_list once after zv_set_basis");
+    const PP pp = pp_view(c);
+    constexpr int ncls = ZV_LMAX_T + 2;
+    static int h_qb[kNaQ], h_qk[kNaQ], h_pos[kNaQ];
+    static unsigned char h_bk[kNaQ];
+    int64_t lo[ncls * ncls + 1] = {};
+    auto order = [&](int P) {
+        const int L = c->h_pair_L[P];
+        return L > ZV_LMAX_T ? ZV_LMAX_T + 1 : L;
+    };
+    for (int64_t t = 0; t < nq; t++) {
+        const int i = std::max(qi[t], qk[t]), k = std::min(qi[t], qk[t]);
+        const int j = std::max(qj[t], ql[t]), l = std::min(qj[t], ql[t]);
+        int P = zv_pidx(i, k), Q = zv_pidx(j, l);
+        if (order(P) < order(Q)) std::swap(P, Q);
+        h_bk[t] = (unsigned char)(order(P) * ncls + order(Q));
+        lo[h_bk[t] + 1]++;
+    }
+    for (int b = 0; b < ncls * ncls; b++) lo[b + 1] += lo[b];
+    {
+        int64_t at[ncls * ncls];
+        for (int b = 0; b < ncls * ncls; b++) at[b] = lo[b];
+        // stable within a bucket: the order of zv_run_gauss_list
+        for (int64_t t = 0; t < nq; t++) {
+            const int i = std::max(qi[t], qk[t]), k = std::min(qi[t], qk[t]);
+            const int j = std::max(qj[t], ql[t]), l = std::min(qj[t], ql[t]);
+            int P = zv_pidx(i, k), Q = zv_pidx(j, l);
+            if (order(P) < order(Q)) std::swap(P, Q);
+            const int64_t x = at[h_bk[t]]++;
+            h_qb[x] = P;
+            h_qk[x] = Q;
+            h_pos[x] = (int)t;
+        }
+    }
+    char *base = sp->second;
+    int *dqb = (int *)base, *dqk = dqb + kNaQ, *dpos = dqk + kNaQ;
+    double *dsorted = (double *)(dpos + kNaQ), *dout = dsorted + kNaQ, *dpart = dout + kNaQ;
+    ZV_HIPCHECK(c, hipMemcpyAsync(dqb, h_qb, sizeof(int) * nq, hipMemcpyHostToDevice, c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(dqk, h_qk, sizeof(int) * nq, hipMemcpyHostToDevice, c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(dpos, h_pos, sizeof(int) * nq, hipMemcpyHostToDevice,
+                                  c->stream));
+    for (int b = 0; b < ncls * ncls; b++) {
+        const int64_t cnt = lo[b + 1] - lo[b];
+        if (!cnt) continue;
+        const int nsplit = class_split(cnt);
+        if (nsplit > 1 && (size_t)(cnt * nsplit) > kNaPart)
+            return zv_fail(c, ZV_ERR_STATE, "split partials exceed the scratch");
+        QSrc S{Q_LIST, dqb + lo[b], dqk + lo[b], cnt};
+        ZV_CHECK(launch_class_raw(c, pp, S, b / ncls, b % ncls, dsorted + lo[b],
+                                  nsplit > 1 ? dpart : nullptr, nsplit));
+    }
+    hipLaunchKernelGGL(k_unpermute, dim3(nblk(nq, 256)), dim3(256), 0, c->stream, nq,
+                       (const int *)dpos, (const double *)dsorted, dout);
+    ZV_HIPCHECK(c, hipGetLastError());
+    ZV_HIPCHECK(c, hipMemcpyAsync(out, dout, sizeof(double) * nq, hipMemcpyDeviceToHost,
+                                  c->stream));
+    ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     return ZV_OK;
 }

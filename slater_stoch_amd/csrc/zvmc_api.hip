@@ -235,6 +235,7 @@ extern "C" int zv_create(int device, zv_context **out)
     }
     for (int k = 0; ok && k < zv_context::NSIDE; k++)
         ok = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (ok) ok = zv_noalloc_setup(c) == ZV_OK;
     if (!ok) {
         zv_destroy(c);
         return ZV_ERR_HIP;
@@ -585,6 +586,27 @@ extern "C" int zv_gauss_ijkl_list(zv_context *c, int64_t nq, const int *qi, cons
     int rc = zv_run_gauss_list(c, nq, qi, qk, qj, ql, out);
     if (rc == ZV_OK) rc = zv_collect_timings(c);
     return rc;
+}
+
+extern "C" int zv_gauss_ijkl_list_noalloc(zv_context *c, int64_t nq, const int *qi,
+                                          const int *qk, const int *qj, const int *ql,
+                                          double *out)
+{
+    ZV_SERIALIZE;
+    ZV_NEED_BASIS(c);
+    if (nq < 0 || (nq > 0 && (!qi || !qk || !qj || !ql || !out)))
+        return zv_fail(c, ZV_ERR_ARG, "bad quartet list");
+    if (nq > ZV_NOALLOC_MAXQ)
+        return zv_fail(c, ZV_ERR_ARG, "%lld quartets: the allocation-free list holds at most %d",
+                       (long long)nq, ZV_NOALLOC_MAXQ);
+    for (int64_t t = 0; t < nq; t++) {
+        if (qi[t] < 1 || qi[t] > c->n || qk[t] < 1 || qk[t] > c->n || qj[t] < 1 ||
+            qj[t] > c->n || ql[t] < 1 || ql[t] > c->n)
+            return zv_fail(c, ZV_ERR_ARG, "quartet %lld has an orbital index outside 1..%d",
+                           (long long)t + 1, c->n);
+    }
+    if (nq == 0) return ZV_OK;
+    return zv_run_gauss_list_noalloc(c, nq, qi, qk, qj, ql, out);
 }
 
 extern "C" int zv_mc_blocks(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed,

@@ -132,6 +132,11 @@ struct zv_context {
     void *blas = nullptr;             // rocBLAS handle (cleanup GEMMs)
 };
 void zv_blas_destroy(zv_context *c);
+// allocation-free list mode (k_eri.hip): device address of its scratch array
+// on the context's device (once per device, at context creation), and the run
+int zv_noalloc_setup(zv_context *c);
+int zv_run_gauss_list_noalloc(zv_context *c, int64_t nq, const int *qi, const int *qk,
+                              const int *qj, const int *ql, double *out);
 
 // ---- helpers (zvmc_api.hip) -------------------------------------------
 int zv_fail(zv_context *c, int code, const char *fmt, ...);

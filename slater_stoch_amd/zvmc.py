@@ -54,7 +54,10 @@ SIGNATURES = {
     "zv_mc_refine": (I32, [VP, I32, I64, U64, U64, I32, F64, VP, VP]),
     "zv_rsvd_clean": (I32, [VP, I32, I32, U64, I32, VP, VP, VP]),
     "zv_davidson_lowest": (I32, [VP, I32, F64, I32, VP, VP, VP, VP, VP]),
+    "zv_gauss_ijkl_list_noalloc": (I32, [VP, I64, VP, VP, VP, VP, VP]),
 }
+
+NOALLOC_MAXQ = 65536
 
 TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS, TUNE_ERI_STREAMS = 0, 1, 2, 3
 
@@ -216,6 +219,15 @@ class Engine:
         out = np.zeros(qi.size)
         self._check(self.lib.zv_gauss_ijkl_list(self.h, qi.size, _p(qi), _p(qk), _p(qj), _p(ql),
                                                 _p(out)))
+        return out
+
+    def gauss_ijkl_noalloc(self, qi, qk, qj, ql):
+        """gauss_ijkl through the allocation-free entry point (at most
+        NOALLOC_MAXQ quartets; the primitive-pair tables must exist)."""
+        qi, qk, qj, ql = [np.ascontiguousarray(a, np.int32) for a in (qi, qk, qj, ql)]
+        out = np.zeros(qi.size)
+        self._check(self.lib.zv_gauss_ijkl_list_noalloc(self.h, qi.size, _p(qi), _p(qk), _p(qj),
+                                                        _p(ql), _p(out)))
         return out
 
     # ------------------------------------------------------------ MC
