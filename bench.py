@@ -40,7 +40,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-from slater_stoch_amd.shells import kernel_unit_pairs, shell_firsts  # noqa: E402
+from slater_stoch_amd.shells import kernel_unit_pairs, mfma_tile_use, shell_firsts  # noqa: E402
 
 METRIC = "sec to 1e-6 Ha stderr on all (ij|kl), mixed-prec vs FP64 GPU vs CPU; MFMA %"
 TARGET = 1e-6
@@ -188,6 +188,18 @@ def roofline_eri(res):
             "flops_per_launch": flops / max(1, nset),
             "heaviest_kernel": kernel_name(heavy[0]),
             "heaviest_kernel_flop_share": heavy[1]["flops"] / flops}
+
+
+def mfma_eligibility(bas):
+    """Why the estimator stays on the vector unit, measured on this basis:
+    the per-shell-pair-pair products that matrix cores would run
+    (shells.mfma_tile_use), as the share of quartets whose two shell pairs
+    both fill a 16-wide tile and the useful share of the tile area with 16 x
+    16 tiles (BF16/FP16 MFMA) and with 4 x 4 tiles (FP64 MFMA)."""
+    s16, u16 = mfma_tile_use(bas, 16)
+    _, u4 = mfma_tile_use(bas, 4)
+    return {"quartets_in_16_wide_blocks": s16, "tile_area_used_16x16": u16,
+            "tile_area_used_4x4": u4}
 
 
 def roofline_estimator(timings, bas, prec, samples):
@@ -510,6 +522,7 @@ def main():
         "host_copy_s": float(np.mean([r["host_copy"] for r in res64])),
         "with_host_copy_s": float(np.mean([r["total"] + r["host_copy"] for r in res64])),
         "mfma_fraction": 0.0,
+        "mfma_eligibility": mfma_eligibility(bas),
         "roofline": roofline_eri(last),
         "roofline_other_phase": roofline_estimator(last["pilot_timings"], bas, "fp64",
                                                    NBL * PILOT),

@@ -263,6 +263,9 @@ int zv_set_verbose(zv_context *ctx, int level);
  *  ZV_TUNE_ERI_STREAMS streams that run the analytic kernel buckets side by
  *                     side (1..5; 1 runs them one after another, which
  *                     times every bucket on its own)
+ *  ZV_TUNE_EST_TILES  estimator tile shapes: 0 the default set, 1 half the
+ *                     units per thread along the longer side (more, smaller
+ *                     tiles for small molecules)
  * Values < 0 give ZV_ERR_ARG; the setting holds for the context's later
  * calls and changes only speed, except ZV_TUNE_EST_SPLIT and
  * ZV_TUNE_MC_CHUNK, which also change the estimator's summation order
@@ -271,6 +274,7 @@ int zv_set_verbose(zv_context *ctx, int level);
 #define ZV_TUNE_MC_CHUNK 1
 #define ZV_TUNE_ERI_ITEMS 2
 #define ZV_TUNE_ERI_STREAMS 3
+#define ZV_TUNE_EST_TILES 4
 #define ZV_TUNE_N 8
 int zv_set_tuning(zv_context *ctx, int key, int64_t value);
 
@@ -281,11 +285,11 @@ int zv_set_tuning(zv_context *ctx, int key, int64_t value);
 int zv_last_timings(const zv_context *ctx, double *ms, int64_t *launches);
 
 /* Per ERI kernel of the last zv_gauss_eri / zv_gauss_ijkl_list call, arrays
- * of 136.  Entries 0..35: component kernels, index 6*LB + LK (LB >= LK the
+ * of 236.  Entries 0..35: component kernels, index 6*LB + LK (LB >= LK the
  * Hermite orders of the two AO pairs, 5 = the runtime-order kernel);
- * entries 36..135: shell kernels, index 36 + 25*v + 5*LU + LL (LU, LL the
+ * entries 36..235: shell kernels, index 36 + 25*v + 5*LU + LL (LU, LL the
  * orders of the uniform-side and lane-side shell pairs; v = 2*(lane batch-1)
- * + mode, mode 0 direct, 1 wave-sum reduction).  ms: kernel milliseconds;
+ * + mode, lane batch 1..4, mode 0 direct, 1 wave-sum reduction).  ms: kernel milliseconds;
  * launches; units: primitive-quartet evaluations (times the lane-side
  * components for shell kernels); flops: algorithmic FP64 flops (Boys values
  * and R table once per primitive quartet, contractions per component).

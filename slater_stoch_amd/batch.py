@@ -102,24 +102,37 @@ def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=400
 
 
 def run_batch(bases, target=1e-6, precision="fp32", workers=1, device=0, seed=0,
-              callback=None, verbose=0, **policy):
+              callback=None, verbose=0, tuning=None, autotune=False, **policy):
     """Run every basis of ``bases``; returns the per-molecule dicts in input
     order.  ``callback(index, result)`` is called as each molecule finishes.
     ``verbose`` is the engine's progress level (2: a line per finished Monte
     Carlo block).  ``policy`` goes to run_molecule (nbl, pilot_npts,
     max_samples, safety, max_rounds, keep_arrays).  ``workers`` must be 1
-    (concurrent contexts are not supported, see the module docstring)."""
+    (concurrent contexts are not supported, see the module docstring).
+
+    ``tuning`` is a table {size class: {knob: value}} (autotune.tune_classes)
+    applied before each molecule of that class; ``autotune=True`` builds it
+    first on this batch, its time counted in the results' "t_tune" of the
+    first molecule run."""
     if int(workers) != 1:
         raise ValueError("run_batch: workers must be 1 (contexts driven from several host "
                          "threads at once are not supported)")
+    from . import autotune as AT
     from .zvmc import Engine
 
     order = sorted(range(len(bases)), key=lambda t: -bases[t].nint)
     results = [None] * len(bases)
     with Engine(device, verbose=verbose) as eng:
+        t_tune = 0.0
+        if autotune and tuning is None:
+            t0 = time.perf_counter()
+            tuning, _ = AT.tune_classes(eng, bases, precision)
+            t_tune = time.perf_counter() - t0
         for t in order:
+            AT.apply(eng, AT.settings_for(tuning, bases[t]))
             r = run_molecule(eng, bases[t], target=target, precision=precision, seed=seed + t,
                              **policy)
+            r["t_tune"], t_tune = t_tune, 0.0
             r["index"] = t
             results[t] = r
             if callback is not None:
@@ -133,6 +146,7 @@ def summary(results, wall):
                 quartets=int(sum(r["nint"] for r in results)),
                 samples=int(sum(r["samples"] for r in results)),
                 quartet_samples=float(sum(float(r["nint"]) * r["samples"] for r in results)),
+                tune_s=float(sum(r.get("t_tune", 0.0) for r in results)),
                 analytic_s=float(sum(r["t_analytic"] for r in results)),
                 mc_s=float(sum(r["t_mc"] for r in results)))
 
@@ -151,6 +165,8 @@ def main(argv=None):
     ap.add_argument("--max-samples", type=int, default=10_000_000)
     ap.add_argument("--max-rounds", type=int, default=4)
     ap.add_argument("--policy", default="refine", choices=["refine", "rerun"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch parameters per size class first (timed in wall_s)")
     ap.add_argument("--verbose", type=int, default=2,
                     help="engine progress level (2: one line per Monte Carlo block)")
     args = ap.parse_args(argv)
@@ -164,10 +180,10 @@ def main(argv=None):
     t0 = time.perf_counter()
     res = run_batch(bases, target=args.target, precision=args.precision, workers=args.workers,
                     callback=show, verbose=args.verbose, max_samples=args.max_samples,
-                    max_rounds=args.max_rounds, policy=args.policy)
+                    max_rounds=args.max_rounds, policy=args.policy, autotune=args.autotune)
     s = summary(res, time.perf_counter() - t0)
     s.update(precision=args.precision, workers=args.workers, target=args.target,
-             policy=args.policy)
+             policy=args.policy, autotune=args.autotune)
     print(json.dumps(s), flush=True)
     return 0
 

@@ -53,6 +53,9 @@ struct PP {
 // F_n(T) = int_0^1 t^2n exp(-T t^2) dt for n = 0..L, as src/rint_sum.F90:
 // T < 1: Taylor series of F_L (20 terms) and downward recursion;
 // T >= 1: F_0 from erf and upward recursion, exp(-T) = 0 beyond T = 80.
+// From T = 36 on erf(sqrt T) = 1 - erfc(6) rounds to 1.0 (erfc(6) = 2.2e-17
+// is below half an ulp of 1), so the call is skipped there: same value,
+// and a wave whose lanes all have T >= 36 (distant pairs) skips the erf.
 template <int L> struct BoysCoef {
     double c[20];
     constexpr BoysCoef() : c()
@@ -81,7 +84,9 @@ template <int L> __device__ inline void boys(double T, double *F)
     } else {
         double u_inv = rsqrt(T);
         double u = T * u_inv;
-        F[0] = ((0.5 * u_inv) * kSqrtPi) * erf(u);
+        double ef = 1.0;
+        if (T < 36.0) ef = erf(u);
+        F[0] = ((0.5 * u_inv) * kSqrtPi) * ef;
         if (L > 0) {
             double e = T > 80.0 ? 0.0 : exp(-T);
             double tri = (0.5 * u_inv) * u_inv;

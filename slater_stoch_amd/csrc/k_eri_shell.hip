@@ -134,14 +134,13 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
     const double *eu = eu_all + sd.euoff[it.su] + (int64_t)it.cu0 * HU;
     const int64_t erow = (int64_t)ncu * HU;
     const int nlc = it.cl1 - it.cl0;
+    // the item's lane components are whole passes of CLB (host-checked)
     for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
         int64_t el[CLB];
         int capl[CLB];
-        bool has[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) {
-            has[j] = cl + j < it.cl1;
-            const int Pl = sd.comp[lb + (has[j] ? cl + j : cl)];
+            const int Pl = sd.comp[lb + cl + j];
             el[j] = pp.eoff[Pl];
             capl[j] = pp.cap[Pl];
         }
@@ -158,7 +157,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
             for (int j = 0; j < CLB; j++)
 #pragma unroll
                 for (int h = 0; h < HL; h++) {
-                    const double e = has[j] ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                    const double e = pp.E[el[j] + (int64_t)h * capl[j] + cc];
                     El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -e : e;
                 }
             for (int a = it.a0; a < it.a1; a++) {
@@ -198,7 +197,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 const double *ea = eu + (int64_t)a * erow;
                 for (int u = 0; u < it.nu; u++) {
                     // independent partial sums keep the FMA chains short
-                    constexpr int NP = CLB == 1 ? 8 : 4;
+                    constexpr int NP = CLB == 1 ? 8 : CLB == 2 ? 4 : 2;
                     double y[CLB][NP];
 #pragma unroll
                     for (int j = 0; j < CLB; j++)
@@ -225,8 +224,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
 #pragma unroll
             for (int j = 0; j < CLB; j++) {
                 const double v = wave_sum(accs[(u * CLB + j) * 64 + lane]);
-                if (lane == 0 && has[j])
-                    part[it.part + (int64_t)u * nlc + (cl + j - it.cl0)] = v;
+                if (lane == 0) part[it.part + (int64_t)u * nlc + (cl + j - it.cl0)] = v;
             }
     }
 }
@@ -243,8 +241,7 @@ constexpr int pow2ceil(int x)
 // X_hu = sum over all lane primitive pairs (every lane chunk, the prefactor
 // folded into the lane E), sums it over the 64 lanes through LDS, and lane u
 // then contracts the sum with the E of its own uniform component u.  One LDS
-// reduction per (a, pass); the uniform E of lane u is loaded before the
-// chunk loop when the registers allow, so its latency hides behind it.
+// reduction per (a, pass).
 template <int LU, int LL, int CLB>
 __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                                                       const ShellItem *__restrict__ items,
@@ -255,7 +252,6 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
     constexpr int NR = CLB * HU; // rows summed over the wave
     constexpr int S = NR | 1;    // odd lane stride of the transpose buffer
-    constexpr bool PREF = HR + CLB * (HL + HU) + HU <= 200;
     __shared__ double lds[64 * S + NR];
     const int lane = threadIdx.x;
     const ShellItem it = items[blockIdx.x];
@@ -265,17 +261,17 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
     const int nl = pp.cnt[repl];
     const int64_t ol = pp.off[repl];
     const double *pu = pu_all + sd.puoff[it.su];
-    const double *eu2 = eu2_all + sd.euoff[it.su] + it.cu0 + lane;
     const int nlc = it.cl1 - it.cl0;
     const bool own = lane < it.nu; // lane owns uniform component cu0 + lane
+    // lanes past the uniform range read component cu0 (in range) and drop it
+    const double *eu2 = eu2_all + sd.euoff[it.su] + it.cu0 + (own ? lane : 0);
+    // the item's lane components are whole passes of CLB (host-checked)
     for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
         int64_t el[CLB];
         int capl[CLB];
-        bool has[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) {
-            has[j] = cl + j < it.cl1;
-            const int Pl = sd.comp[lb + (has[j] ? cl + j : cl)];
+            const int Pl = sd.comp[lb + cl + j];
             el[j] = pp.eoff[Pl];
             capl[j] = pp.cap[Pl];
         }
@@ -286,11 +282,6 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
             const double *pa = pu + (int64_t)a * 8;
             const double p = pa[0], ip = pa[1], Px = pa[2], Py = pa[3], Pz = pa[4], cb = pa[5];
             const double *ea = eu2 + (int64_t)a * HU * ncu;
-            double ev[HU];
-            if constexpr (PREF) {
-#pragma unroll
-                for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
-            }
             double Xa[CLB][HU];
 #pragma unroll
             for (int j = 0; j < CLB; j++)
@@ -318,8 +309,7 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 for (int j = 0; j < CLB; j++)
 #pragma unroll
                     for (int h = 0; h < HL; h++) {
-                        const double e =
-                            has[j] ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                        const double e = pp.E[el[j] + (int64_t)h * capl[j] + cc];
                         El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -(e * f) : e * f;
                     }
 #pragma unroll
@@ -333,10 +323,11 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                                                    kH.v[hu] + kH.v[hl])],
                                             Xa[j][hu]);
             }
-            if constexpr (!PREF) {
+            // the lane's uniform E, loaded here so that its latency hides
+            // behind the LDS reduction below
+            double ev[HU];
 #pragma unroll
-                for (int hu = 0; hu < HU; hu++) ev[hu] = own ? ea[(int64_t)hu * ncu] : 0.0;
-            }
+            for (int hu = 0; hu < HU; hu++) ev[hu] = ea[(int64_t)hu * ncu];
 #pragma unroll
             for (int j = 0; j < CLB; j++)
 #pragma unroll
@@ -375,7 +366,7 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
         }
 #pragma unroll
         for (int j = 0; j < CLB; j++)
-            if (own && has[j]) part[it.part + (int64_t)lane * nlc + (cl + j - it.cl0)] = acc[j];
+            if (own) part[it.part + (int64_t)lane * nlc + (cl + j - it.cl0)] = acc[j];
     }
 }
 
@@ -408,39 +399,62 @@ __global__ void k_zero(double *x, int64_t n)
 }
 
 // Largest lane-component batch of each (LU, LL) class, direct and reduction
-// mode: two wherever the shared R table and the doubled lane registers fit
-// one wave per SIMD without spilling.  Items with a single lane component
-// use batch 1.
-constexpr int kShellClb[5][5] = {{2, 2, 2, 2, 2},
-                                 {2, 2, 2, 2, 2},
-                                 {2, 2, 2, 2, 1},
-                                 {2, 2, 2, 2, 1},
-                                 {2, 2, 2, 1, 1}};
-constexpr int kShellClbRed[5][5] = {{2, 2, 2, 2, 2},
-                                    {2, 2, 2, 2, 1},
-                                    {2, 2, 2, 2, 1},
-                                    {2, 2, 2, 1, 1},
-                                    {2, 2, 1, 1, 1}};
+// mode: the lane components of a shell pair run in passes of this many (and
+// one pass of the remainder); the components of one pass share the Boys
+// values and the R table of every primitive quartet, so wider passes redo
+// them less often.  Chosen so that the R table, the pass's lane E and its X
+// sums stay in the register file without spilling, and by shell-pair sizes:
+// an order-0 pair has 1 component, order 1 has 3 (p x s), order 2 has 9
+// (p x p) or 6 (d x s), so 3 covers them in as few passes as 4 does with
+// fewer registers.
+constexpr int kClbMax = 4;
+constexpr int kShellClb[5][5] = {{1, 3, 3, 4, 4},
+                                 {1, 3, 3, 4, 3},
+                                 {1, 3, 3, 3, 1},
+                                 {1, 3, 3, 2, 1},
+                                 {1, 3, 2, 1, 1}};
+constexpr int kShellClbRed[5][5] = {{1, 3, 3, 4, 2},
+                                    {1, 3, 3, 3, 1},
+                                    {1, 3, 3, 2, 1},
+                                    {1, 3, 3, 1, 1},
+                                    {1, 2, 1, 1, 1}};
 
+template <int LU, int LL, int B>
+void launch_shell_b(bool red, unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
+                    const ShellItem *items, double *part)
+{
+    if (red)
+        hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B>), dim3(grid), dim3(64), 0, st, pp, sd,
+                           items, sd.pu, sd.eu2, part);
+    else
+        hipLaunchKernelGGL((k_eri_shell<LU, LL, B>), dim3(grid), dim3(64), 0, st, pp, sd, items,
+                           sd.pu, sd.eu, part);
+}
+
+// instances up to the class's batch limit only (the larger ones are never
+// planned); false for a batch outside it
 template <int LU, int LL>
-void launch_shell_t(int clb, bool red, unsigned grid, hipStream_t st, const PP &pp,
+bool launch_shell_t(int clb, bool red, unsigned grid, hipStream_t st, const PP &pp,
                     const SPD &sd, const ShellItem *items, double *part)
 {
-    if (red) {
-        if (clb == 2)
-            hipLaunchKernelGGL((k_eri_shell_red<LU, LL, 2>), dim3(grid), dim3(64), 0, st, pp, sd,
-                               items, sd.pu, sd.eu2, part);
-        else
-            hipLaunchKernelGGL((k_eri_shell_red<LU, LL, 1>), dim3(grid), dim3(64), 0, st, pp, sd,
-                               items, sd.pu, sd.eu2, part);
-    } else {
-        if (clb == 2)
-            hipLaunchKernelGGL((k_eri_shell<LU, LL, 2>), dim3(grid), dim3(64), 0, st, pp, sd,
-                               items, sd.pu, sd.eu, part);
-        else
-            hipLaunchKernelGGL((k_eri_shell<LU, LL, 1>), dim3(grid), dim3(64), 0, st, pp, sd,
-                               items, sd.pu, sd.eu, part);
+    constexpr int MR = kShellClbRed[LU][LL], MD = kShellClb[LU][LL];
+    if (clb < 1 || clb > (red ? MR : MD)) return false;
+    switch (clb) {
+    case 1: launch_shell_b<LU, LL, 1>(red, grid, st, pp, sd, items, part); return true;
+    case 2:
+        if constexpr (MR >= 2 || MD >= 2)
+            launch_shell_b<LU, LL, 2>(red, grid, st, pp, sd, items, part);
+        return true;
+    case 3:
+        if constexpr (MR >= 3 || MD >= 3)
+            launch_shell_b<LU, LL, 3>(red, grid, st, pp, sd, items, part);
+        return true;
+    case 4:
+        if constexpr (MR >= 4 || MD >= 4)
+            launch_shell_b<LU, LL, 4>(red, grid, st, pp, sd, items, part);
+        return true;
     }
+    return false;
 }
 
 template <int LU>
@@ -448,11 +462,11 @@ bool launch_shell_u(int LL, int clb, bool red, unsigned grid, hipStream_t st, co
                     const SPD &sd, const ShellItem *items, double *part)
 {
     switch (LL) {
-    case 0: launch_shell_t<LU, 0>(clb, red, grid, st, pp, sd, items, part); return true;
-    case 1: launch_shell_t<LU, 1>(clb, red, grid, st, pp, sd, items, part); return true;
-    case 2: launch_shell_t<LU, 2>(clb, red, grid, st, pp, sd, items, part); return true;
-    case 3: launch_shell_t<LU, 3>(clb, red, grid, st, pp, sd, items, part); return true;
-    case 4: launch_shell_t<LU, 4>(clb, red, grid, st, pp, sd, items, part); return true;
+    case 0: return launch_shell_t<LU, 0>(clb, red, grid, st, pp, sd, items, part);
+    case 1: return launch_shell_t<LU, 1>(clb, red, grid, st, pp, sd, items, part);
+    case 2: return launch_shell_t<LU, 2>(clb, red, grid, st, pp, sd, items, part);
+    case 3: return launch_shell_t<LU, 3>(clb, red, grid, st, pp, sd, items, part);
+    case 4: return launch_shell_t<LU, 4>(clb, red, grid, st, pp, sd, items, part);
     }
     return false;
 }
@@ -499,19 +513,28 @@ double finish_red(int LU, int clb)
     return 64.0 * (4.0 * NR + 128.0 * std::ceil(NR / 64.0)) + 40000.0;
 }
 
+// bucket b = 25 (2 (clb - 1) + red) + 5 LU + LL
+constexpr int kNB = 25 * 2 * kClbMax;
+
 struct Plan {
-    // bucket b = 25 (2 (clb - 1) + red) + 5 LU + LL at [lo[b], lo[b + 1])
+    // bucket b at [lo[b], lo[b + 1])
     std::vector<ShellItem> items;
-    int lo[101] = {};
+    int lo[kNB + 1] = {};
     std::vector<ShellGroup> groups;
     int64_t npart = 0;
-    double units[100] = {}, flops[100] = {}; // per bucket
+    double units[kNB] = {}, flops[kNB] = {}; // per bucket
 };
 
-// lane components run in batches of two (even part) and one (odd remainder)
-inline int paired(int LU, int LL, int ncl, bool red)
+// lane components [0, full) run in passes of the class's batch limit, the
+// remainder [full, ncl) in one pass
+inline int lane_batch(int LU, int LL, bool red)
 {
-    return (red ? kShellClbRed[LU][LL] : kShellClb[LU][LL]) == 2 ? ncl & ~1 : 0;
+    return red ? kShellClbRed[LU][LL] : kShellClb[LU][LL];
+}
+inline int full_passes_end(int LU, int LL, int ncl, bool red)
+{
+    const int b = lane_batch(LU, LL, red);
+    return ncl / b * b;
 }
 
 // uniform-component ranges of one orientation: ranges of kMaxURed (reduction
@@ -539,9 +562,9 @@ double item_cost(const zv_context *c, int su, int sl)
     for (int cu0 = 0; cu0 < ncu; cu0 += step) {
         const int nu = std::min(step, ncu - cu0);
         const bool red = uniform_red(ncu, nu);
-        const int n2 = paired(LU, LL, ncl, red);
-        cost += n2 / 2 * pass_cost(LU, LL, 2, nu, red, npu, lanes) +
-                (ncl - n2) * pass_cost(LU, LL, 1, nu, red, npu, lanes);
+        const int bm = lane_batch(LU, LL, red), n2 = full_passes_end(LU, LL, ncl, red);
+        cost += n2 / bm * pass_cost(LU, LL, bm, nu, red, npu, lanes);
+        if (ncl > n2) cost += pass_cost(LU, LL, ncl - n2, nu, red, npu, lanes);
     }
     return cost;
 }
@@ -576,9 +599,9 @@ template <typename F> void run_threads(int nt, F &&f)
 // would make them but with partial-sum offsets from 0.
 struct PlanPart {
     std::vector<ShellGroup> groups;
-    std::vector<ShellItem> tmp[100];
-    std::vector<float> tcost[100];
-    double units[100] = {}, flops[100] = {};
+    std::vector<ShellItem> tmp[kNB];
+    std::vector<float> tcost[kNB];
+    double units[kNB] = {}, flops[kNB] = {};
     int64_t npart = 0;
 };
 
@@ -602,11 +625,12 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
         for (int cu0 = 0; cu0 < ncu; cu0 += step) {
             const int nu = std::min(step, ncu - cu0);
             const bool red = uniform_red(ncu, nu);
-            const int n2 = paired(LU, LL, ncl, red);
-            // segment [0, n2) in passes of two components, [n2, ncl) in passes of one
+            const int bm = lane_batch(LU, LL, red), n2 = full_passes_end(LU, LL, ncl, red);
+            // segment [0, n2) in passes of bm components, [n2, ncl) in one pass
             for (int seg = 0; seg < 2; seg++) {
-                const int clb = seg == 0 ? 2 : 1;
+                const int clb = seg == 0 ? bm : ncl - n2;
                 const int s0 = seg == 0 ? 0 : n2, s1 = seg == 0 ? n2 : ncl;
+                if (clb <= 0) continue;
                 const int npass = (s1 - s0) / clb;
                 if (npass <= 0) continue;
                 const double wp = pass_cost(LU, LL, clb, nu, red, npu, lanes); // per pass
@@ -679,18 +703,18 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
     });
     // join the runs in order: partial-sum offsets shifted by the runs before
     pl.npart = 0;
-    size_t ng = 0, nb_items[100] = {};
+    size_t ng = 0, nb_items[kNB] = {};
     for (const PlanPart &pp : parts) {
         ng += pp.groups.size();
-        for (int b = 0; b < 100; b++) nb_items[b] += pp.tmp[b].size();
+        for (int b = 0; b < kNB; b++) nb_items[b] += pp.tmp[b].size();
     }
     pl.groups.reserve(ng);
     size_t at = 0;
-    for (int b = 0; b < 100; b++) {
+    for (int b = 0; b < kNB; b++) {
         pl.lo[b] = (int)at;
         at += nb_items[b];
     }
-    pl.lo[100] = (int)at;
+    pl.lo[kNB] = (int)at;
     pl.items.resize(at);
     std::vector<int64_t> off(nt, 0);
     for (int t = 0; t < nt; t++) {
@@ -700,7 +724,7 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
             g.part += off[t];
             pl.groups.push_back(g);
         }
-        for (int b = 0; b < 100; b++) {
+        for (int b = 0; b < kNB; b++) {
             pl.units[b] += pp.units[b];
             pl.flops[b] += pp.flops[b];
         }
@@ -715,7 +739,7 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         std::vector<uint8_t> bin;
         std::vector<const ShellItem *> src;
         std::vector<float> cst;
-        for (int b = t; b < 100; b += nt) {
+        for (int b = t; b < kNB; b += nt) {
             const size_t n = nb_items[b];
             if (!n) continue;
             src.clear();
@@ -837,18 +861,27 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
     // Largest planned cost first, each to the stream with the least planned
     // cost so far.
     // bucket b = 25 v + 5 LU + LL, v = 2 (clb - 1) + red: statistics slot 36 + b
-    for (int b = 0; b < 100; b++) {
-        if ((b / 25) & 1) continue; // reduction mode
-        for (int t = pl.lo[b]; t < pl.lo[b + 1]; t++)
-            if (pl.items[t].nu < 1 || pl.items[t].nu > kDirectMaxU)
+    static_assert(36 + kNB <= ZV_NSTAT, "statistics slots");
+    // the kernels assume whole lane passes (no guard per lane component) and,
+    // in direct mode, at most kDirectMaxU uniform components
+    for (int b = 0; b < kNB; b++) {
+        const int clb = b / 50 + 1;
+        const bool red = ((b / 25) & 1) != 0;
+        for (int t = pl.lo[b]; t < pl.lo[b + 1]; t++) {
+            const ShellItem &it = pl.items[t];
+            if (it.cl1 <= it.cl0 || (it.cl1 - it.cl0) % clb != 0)
+                return zv_fail(c, ZV_ERR_STATE, "shell item with lane components [%d, %d) "
+                               "in passes of %d", it.cl0, it.cl1, clb);
+            if (!red && (it.nu < 1 || it.nu > kDirectMaxU))
                 return zv_fail(c, ZV_ERR_STATE, "direct shell item with %d uniform components",
-                               pl.items[t].nu);
+                               it.nu);
+        }
     }
     constexpr int NST = 1 + zv_context::NSIDE;
     hipStream_t sts[NST] = {c->stream};
     for (int k = 0; k < zv_context::NSIDE; k++) sts[1 + k] = c->side[k];
-    int order[100], nb = 0;
-    for (int b = 0; b < 100; b++)
+    int order[kNB], nb = 0;
+    for (int b = 0; b < kNB; b++)
         if (pl.lo[b + 1] > pl.lo[b]) order[nb++] = b;
     std::sort(order, order + nb, [&](int x, int y) { return pl.flops[x] > pl.flops[y]; });
     double load[NST] = {};

@@ -15,6 +15,7 @@
 // P >= Q (P = electron-1 pair), which is the reference's set of canonical
 // quartets; results are scattered to the reference kcp order at the end.
 #include <algorithm>
+#include <array>
 #include <cmath>
 
 #include "zv_math.h"
@@ -295,10 +296,11 @@ __global__ void k_dens_group(Basis B, PairTab Tb, int npp, const int *__restrict
                              const int *__restrict__ sp_first, int npos, int nsp, int Mv, int Mc,
                              const double *__restrict__ r1, const double *__restrict__ r2,
                              const double *__restrict__ pi0, T *__restrict__ comp,
-                             T *__restrict__ grp)
+                             T *__restrict__ grp, const int *__restrict__ sp_list)
 {
+    // sp_list (may be null): the shell pairs to evaluate, one per grid row
     const int mblocks = (Mc + blockDim.x - 1) / blockDim.x;
-    const int sp = blockIdx.x / mblocks;
+    const int sp = sp_list ? sp_list[blockIdx.x / mblocks] : blockIdx.x / mblocks;
     const int m = (blockIdx.x % mblocks) * blockDim.x + threadIdx.x;
     if (m >= Mc) return;
     const int pos0 = sp_first[sp], nc = sp_first[sp + 1] - pos0;
@@ -667,6 +669,108 @@ __global__ __launch_bounds__(256) void k_est_group(
     }
 }
 
+// ------------------------------------------------------------------ unit-pair estimator
+// The estimator for a short list of unit pairs (targeted refinement): one
+// workgroup per (row unit, column unit) pair, its threads over the samples of
+// the chunk, with the kernel value and products of k_est_group (K in T,
+// K A1 A2 + K B1 B2), summed per thread in FP64 and over the workgroup at
+// the end; an element is written only when its row pair P >= its column pair
+// Q, as in k_est_group.  up[6 u ..]: row position, row shell pair, row
+// components, column position, column shell pair, column components.
+template <typename T>
+__global__ __launch_bounds__(256) void k_est_pairs(const T *__restrict__ comp,
+                                                   const T *__restrict__ grp,
+                                                   const int *__restrict__ up,
+                                                   const int *__restrict__ pos_pair, int Mv,
+                                                   int Mc, int npos, int nsp_pad,
+                                                   double *__restrict__ acc)
+{
+    const int *u = up + 6 * blockIdx.x;
+    const int rpos = u[0], rsp = u[1], nr = u[2], cpos = u[3], csp = u[4], nc = u[5];
+    const size_t cplane = (size_t)npos * Mc, gplane = (size_t)nsp_pad * Mc;
+    double s[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) s[i][j] = 0.0;
+    for (int m = threadIdx.x; m < Mv; m += blockDim.x) {
+        const T dx = grp[(size_t)rsp * Mc + m] - grp[3 * gplane + (size_t)csp * Mc + m];
+        const T dy = grp[gplane + (size_t)rsp * Mc + m] - grp[4 * gplane + (size_t)csp * Mc + m];
+        const T dz =
+            grp[2 * gplane + (size_t)rsp * Mc + m] - grp[5 * gplane + (size_t)csp * Mc + m];
+        const T K = rsq(fma(dz, dz, fma(dy, dy, dx * dx)));
+        T a2[6], b2[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            a2[j] = j < nc ? comp[2 * cplane + (size_t)(cpos + j) * Mc + m] : (T)0;
+            b2[j] = j < nc ? comp[3 * cplane + (size_t)(cpos + j) * Mc + m] : (T)0;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            if (i >= nr) break;
+            const T ka = K * comp[(size_t)(rpos + i) * Mc + m];
+            const T kb = K * comp[cplane + (size_t)(rpos + i) * Mc + m];
+#pragma unroll
+            for (int j = 0; j < 6; j++) s[i][j] += (double)fma(kb, b2[j], ka * a2[j]);
+        }
+    }
+    __shared__ double red[4][36];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            double v = s[i][j];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[w][6 * i + j] = v;
+        }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 36) {
+        const int i = t / 6, j = t % 6;
+        if (i < nr && j < nc) {
+            const int P = pos_pair[rpos + i], Q = pos_pair[cpos + j];
+            if (P >= Q) acc[zv_tri(P, Q)] += (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        }
+    }
+}
+
+// block end for the selected quartets only (their triangle slots tri[x])
+__global__ void k_sel_block_end(int64_t nsel, const int64_t *__restrict__ tri, double f,
+                                double *accS, double *s1, double *s2)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsel) return;
+    const double v = accS[tri[x]] * f;
+    accS[tri[x]] = 0.0;
+    s1[x] += v;
+    s2[x] += v * v;
+}
+
+// mean and standard error of the selected quartets (k_final_stats) into the
+// kept results at their reference positions
+__global__ void k_sel_final(int64_t nsel, int nbl, const double *__restrict__ s1,
+                            const double *__restrict__ s2, const int64_t *__restrict__ list,
+                            double *res_mean, double *res_err)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsel) return;
+    const double m1 = s1[x] / nbl, m2 = s2[x] / nbl;
+    res_mean[list[x]] = m1;
+    res_err[list[x]] = sqrt(fabs(m2 - m1 * m1)) / sqrt((double)nbl);
+}
+
+__global__ void k_sel_zero(int64_t nsel, const int64_t *__restrict__ tri, double *accS,
+                           double *s1, double *s2)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsel) return;
+    accS[tri[x]] = 0.0;
+    s1[x] = 0.0;
+    s2[x] = 0.0;
+}
+
 // ------------------------------------------------------------------ statistics
 // End of a block (src/two_e_slater.F90:463-481): x = e_S/npts - e_G/npts,
 // sums of x and x^2, accumulators cleared.
@@ -933,9 +1037,15 @@ static int pick_split(zv_context *c, int Mc, int precision, int64_t tiles)
 // unit classes: 1, 3 or 6 components of one shell pair
 constexpr int kUnitRows[3] = {1, 3, 6};
 // units per thread (TR row units, TC column units) for each pair of classes
-constexpr int kUnitsPerThread[3][3][2] = {{{4, 4}, {4, 2}, {4, 1}},
-                                          {{2, 4}, {2, 2}, {2, 1}},
-                                          {{1, 4}, {1, 2}, {1, 1}}};
+// Two tile-shape sets (ZV_TUNE_EST_TILES): set 1 halves the units per thread
+// along the longer side, four times fewer units per tile for the narrow
+// classes: more tiles (and less sample splitting) for small molecules.
+constexpr int kUnitsPerThread[2][3][3][2] = {{{{4, 4}, {4, 2}, {4, 1}},
+                                             {{2, 4}, {2, 2}, {2, 1}},
+                                             {{1, 4}, {1, 2}, {1, 1}}},
+                                            {{{2, 2}, {2, 1}, {2, 1}},
+                                             {{1, 2}, {1, 1}, {1, 1}},
+                                             {{1, 2}, {1, 1}, {1, 1}}}};
 
 // Units (greedy split of each shell pair's components into 6, 3 and 1),
 // ordered by the first shell of their shell pair, and the tiles of every
@@ -944,7 +1054,9 @@ constexpr int kUnitsPerThread[3][3][2] = {{{4, 4}, {4, 2}, {4, 1}},
 // when I >= J.
 static int group_ready(zv_context *c)
 {
-    if (c->grp_ready) return ZV_OK;
+    const int set = c->tune[ZV_TUNE_EST_TILES] == 1 ? 1 : 0;
+    if (c->grp_ready && c->grp_tileset == set) return ZV_OK;
+    c->grp_tileset = set;
     const int nsp = c->nsp, npair = c->npair;
     std::vector<int> sp_I(nsp);
     for (int I = 0; I < c->nshell; I++)
@@ -989,7 +1101,8 @@ static int group_ready(zv_context *c)
     if (g.size() & 1) g.push_back(0); // int2 alignment of the tiles
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) {
-            const int RU = 16 * kUnitsPerThread[a][b][0], CU = 16 * kUnitsPerThread[a][b][1];
+            const int RU = 16 * kUnitsPerThread[set][a][b][0];
+            const int CU = 16 * kUnitsPerThread[set][a][b][1];
             const int nr = c->grp_nunit[a], ncl = c->grp_nunit[b];
             const int nrt = (nr + RU - 1) / RU, nct = (ncl + CU - 1) / CU;
             c->grp_tile_off[3 * a + b] = (int64_t)g.size() / 2;
@@ -1037,16 +1150,9 @@ static void launch_group_t(zv_context *c, hipStream_t st, unsigned ntile, const 
 // alone, so they run side by side on the main and the side streams (they
 // write disjoint quartets) and are joined back into the main stream.  With
 // nsplit > 1 every tile is also split over the samples of the chunk.
-// A subset of the tiles of every pair of unit classes (device lists), for
-// runs that refine only some quartets.
-struct TileSel {
-    const int2 *tiles[9];
-    int ntile[9];
-};
-
 template <typename T, bool BF>
 static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int Mround, int Mc,
-                               int nsplit, double *acc, double *mpart, const TileSel *sel)
+                               int nsplit, double *acc, double *mpart)
 {
     constexpr int SUB = GroupSub<T>::SUB;
     const int nstage = Mround / SUB;
@@ -1062,24 +1168,22 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
     // largest launches first
     int order[9];
     for (int k = 0; k < 9; k++) order[k] = k;
-    auto count = [&](int k) { return sel ? sel->ntile[k] : c->grp_ntile[k]; };
-    std::sort(order, order + 9, [&](int x, int y) { return count(x) > count(y); });
+    std::sort(order, order + 9, [&](int x, int y) { return c->grp_ntile[x] > c->grp_ntile[y]; });
     int nl = 0;
     for (int oi = 0; oi < 9; oi++) {
         const int a = order[oi] / 3, b = order[oi] % 3;
         {
-            const unsigned nt = (unsigned)count(3 * a + b);
+            const unsigned nt = (unsigned)c->grp_ntile[3 * a + b];
             if (!nt) continue;
             hipStream_t st = sts[nl++ % (1 + zv_context::NSIDE)];
             const int *urp = g + c->grp_unit_off[a], *urs = urp + c->grp_nunit[a];
             const int *ucp = g + c->grp_unit_off[b], *ucs = ucp + c->grp_nunit[b];
             const int nur = c->grp_nunit[a], nuc = c->grp_nunit[b];
-            const int2 *tiles =
-                sel ? sel->tiles[3 * a + b] : (const int2 *)g + c->grp_tile_off[3 * a + b];
+            const int2 *tiles = (const int2 *)g + c->grp_tile_off[3 * a + b];
 #define ZV_GROUP(NR, NC, TR, TC)                                                                  \
     launch_group_t<T, BF, NR, NC, TR, TC>(c, st, nt, comp, grp, urp, urs, nur, ucp, ucs, nuc, tiles, \
                                       pos_pair, Mround, nsplit, mspan, Mc, acc, mpart)
-            switch (3 * a + b) {
+            switch (9 * c->grp_tileset + 3 * a + b) {
             case 0: ZV_GROUP(1, 1, 4, 4); break;
             case 1: ZV_GROUP(1, 3, 4, 2); break;
             case 2: ZV_GROUP(1, 6, 4, 1); break;
@@ -1089,6 +1193,15 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
             case 6: ZV_GROUP(6, 1, 1, 4); break;
             case 7: ZV_GROUP(6, 3, 1, 2); break;
             case 8: ZV_GROUP(6, 6, 1, 1); break;
+            case 9: ZV_GROUP(1, 1, 2, 2); break;
+            case 10: ZV_GROUP(1, 3, 2, 1); break;
+            case 11: ZV_GROUP(1, 6, 2, 1); break;
+            case 12: ZV_GROUP(3, 1, 1, 2); break;
+            case 13: ZV_GROUP(3, 3, 1, 1); break;
+            case 14: ZV_GROUP(3, 6, 1, 1); break;
+            case 15: ZV_GROUP(6, 1, 1, 2); break;
+            case 16: ZV_GROUP(6, 3, 1, 1); break;
+            case 17: ZV_GROUP(6, 6, 1, 1); break;
             }
 #undef ZV_GROUP
             ZV_HIPCHECK(c, hipGetLastError());
@@ -1101,11 +1214,11 @@ static int run_group_estimator(zv_context *c, const T *comp, const T *grp, int M
     return ZV_OK;
 }
 
-// nbl blocks of npts samples over all tiles (sel null) or a subset; the mean
-// is left in accS and the standard error in b_out2, both in triangle order.
+// nbl blocks of npts samples over every quartet; the mean is left in accS
+// and the standard error in b_out2, both in triangle order.
 static int run_mc_core(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed,
                        uint64_t stream, int precision, const double *r1, const double *r2,
-                       const double *pi0, double *block_means, const TileSel *sel)
+                       const double *pi0, double *block_means)
 {
     const int64_t ntri = c->nint;
     const int np = c->npair, npp = c->npair_pad;
@@ -1126,7 +1239,7 @@ static int run_mc_core(zv_context *c, int nbl, int64_t npts, int mode, uint64_t 
     ZV_CHECK(zv_ensure(c, c->b_s2, sizeof(double) * ntri));
     if (block_means) ZV_CHECK(zv_ensure(c, c->b_blk, sizeof(double) * ntri));
     int64_t ntiles = 0;
-    for (int k = 0; k < 9; k++) ntiles += sel ? sel->ntile[k] : c->grp_ntile[k];
+    for (int k = 0; k < 9; k++) ntiles += c->grp_ntile[k];
     const int nsplit = precision == ZV_PREC_FP64_PARITY ? 1 : pick_split(c, Mc, precision, ntiles);
     if (nsplit > 1) ZV_CHECK(zv_ensure(c, c->b_mpart, sizeof(double) * (nsplit - 1) * ntri));
     double *mpart = nsplit > 1 ? (double *)c->b_mpart.p : nullptr;
@@ -1187,28 +1300,28 @@ static int run_mc_core(zv_context *c, int nbl, int64_t npts, int mode, uint64_t 
                         PhaseTimer tm(c, PH_DENS);
                         hipLaunchKernelGGL(k_dens_group<float>, grid, dim3(256), 0, c->stream, B,
                                            PT, npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1,
-                                           d_r2, d_pi0, comp, grp);
+                                           d_r2, d_pi0, comp, grp, (const int *)nullptr);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
                     if (precision == ZV_PREC_BF16)
                         ZV_CHECK((run_group_estimator<float, true>(c, comp, grp, Mround, Mc,
-                                                                   nsplit, accS, mpart, sel)));
+                                                                   nsplit, accS, mpart)));
                     else
                         ZV_CHECK((run_group_estimator<float, false>(c, comp, grp, Mround, Mc,
-                                                                    nsplit, accS, mpart, sel)));
+                                                                    nsplit, accS, mpart)));
                 } else {
                     double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
                     {
                         PhaseTimer tm(c, PH_DENS);
                         hipLaunchKernelGGL(k_dens_group<double>, grid, dim3(256), 0, c->stream, B,
                                            PT, npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1,
-                                           d_r2, d_pi0, comp, grp);
+                                           d_r2, d_pi0, comp, grp, (const int *)nullptr);
                         ZV_HIPCHECK(c, hipGetLastError());
                     }
                     PhaseTimer tm(c, PH_EST);
                     ZV_CHECK((run_group_estimator<double, false>(c, comp, grp, Mround, Mc,
-                                                                 nsplit, accS, mpart, sel)));
+                                                                 nsplit, accS, mpart)));
                 }
             }
         }
@@ -1244,8 +1357,7 @@ int zv_run_mc(zv_context *c, int nbl, int64_t npts, int mode, uint64_t seed, uin
               int precision, const double *r1, const double *r2, const double *pi0, double *mean,
               double *err, double *block_means)
 {
-    ZV_CHECK(run_mc_core(c, nbl, npts, mode, seed, stream, precision, r1, r2, pi0, block_means,
-                         nullptr));
+    ZV_CHECK(run_mc_core(c, nbl, npts, mode, seed, stream, precision, r1, r2, pi0, block_means));
     ZV_CHECK(zv_keep_result(c, (const double *)c->b_accS.p, ZV_RESULT_MEAN, mean));
     ZV_CHECK(zv_keep_result(c, (const double *)c->b_out2.p, ZV_RESULT_STDERR, err));
     return ZV_OK;
@@ -1337,47 +1449,122 @@ int zv_run_mc_refine(zv_context *c, int nbl, int64_t npts, uint64_t seed, uint64
                        (const int64_t *)d_list, (const int64_t *)c->d_kbase, c->n, d_pr, d_pc,
                        d_tri);
     ZV_HIPCHECK(c, hipGetLastError());
-    // 2. the tiles holding them, per pair of unit classes
+    // 2. the unit pairs holding them (row unit of P, column unit of Q) and
+    // the shell pairs those units belong to
     std::vector<int> hpr(nsel), hpc(nsel);
     ZV_HIPCHECK(c, hipMemcpyAsync(hpr.data(), d_pr, sizeof(int) * nsel, hipMemcpyDeviceToHost,
                                   c->stream));
     ZV_HIPCHECK(c, hipMemcpyAsync(hpc.data(), d_pc, sizeof(int) * nsel, hipMemcpyDeviceToHost,
                                   c->stream));
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
-    std::vector<int64_t> keys[9];
-    for (int64_t x = 0; x < nsel; x++) {
-        const int a = c->h_pair_ucls[hpr[x]], b = c->h_pair_ucls[hpc[x]];
-        const int RU = 16 * kUnitsPerThread[a][b][0], CU = 16 * kUnitsPerThread[a][b][1];
-        const int64_t rt = c->h_pair_uidx[hpr[x]] / RU, ct = c->h_pair_uidx[hpc[x]] / CU;
-        keys[3 * a + b].push_back((rt << 32) | ct);
+    std::vector<std::array<int, 4>> pairs(nsel);
+    for (int64_t x = 0; x < nsel; x++)
+        pairs[x] = {c->h_pair_ucls[hpr[x]], c->h_pair_uidx[hpr[x]], c->h_pair_ucls[hpc[x]],
+                    c->h_pair_uidx[hpc[x]]};
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    const int nup = (int)pairs.size();
+    const std::vector<int> &g = c->h_gmeta;
+    std::vector<int> hup(6 * (size_t)nup), hsp;
+    for (int x = 0; x < nup; x++) {
+        const int cr = pairs[x][0], ur = pairs[x][1], cc = pairs[x][2], uc = pairs[x][3];
+        const int *rtab = g.data() + c->grp_unit_off[cr], *ctab = g.data() + c->grp_unit_off[cc];
+        int *o = hup.data() + 6 * (size_t)x;
+        o[0] = rtab[ur];
+        o[1] = rtab[c->grp_nunit[cr] + ur];
+        o[2] = kUnitRows[cr];
+        o[3] = ctab[uc];
+        o[4] = ctab[c->grp_nunit[cc] + uc];
+        o[5] = kUnitRows[cc];
+        hsp.push_back(o[1]);
+        hsp.push_back(o[4]);
     }
-    std::vector<int> htiles;
-    int64_t toff[9];
-    TileSel sel;
-    for (int k = 0; k < 9; k++) {
-        std::sort(keys[k].begin(), keys[k].end());
-        keys[k].erase(std::unique(keys[k].begin(), keys[k].end()), keys[k].end());
-        toff[k] = (int64_t)htiles.size() / 2;
-        for (int64_t key : keys[k]) {
-            htiles.push_back((int)(key >> 32));
-            htiles.push_back((int)(key & 0xffffffff));
-        }
-        sel.ntile[k] = (int)keys[k].size();
-    }
-    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(int) * std::max<size_t>(htiles.size(), 2)));
-    ZV_HIPCHECK(c, hipMemcpyAsync(c->b_part.p, htiles.data(), sizeof(int) * htiles.size(),
+    std::sort(hsp.begin(), hsp.end());
+    hsp.erase(std::unique(hsp.begin(), hsp.end()), hsp.end());
+    const int nspl = (int)hsp.size();
+    ZV_CHECK(zv_ensure(c, c->b_part, sizeof(int) * (hup.size() + hsp.size())));
+    int *d_up = (int *)c->b_part.p, *d_spl = d_up + hup.size();
+    ZV_HIPCHECK(c, hipMemcpyAsync(d_up, hup.data(), sizeof(int) * hup.size(),
                                   hipMemcpyHostToDevice, c->stream));
-    for (int k = 0; k < 9; k++) sel.tiles[k] = (const int2 *)c->b_part.p + toff[k];
-    // 3. a run over those tiles, merged into the kept results for the
-    // selected quartets only
-    ZV_CHECK(run_mc_core(c, nbl, npts, ZV_SAMPLES_PHILOX, seed, stream, precision, nullptr,
-                         nullptr, nullptr, nullptr, &sel));
+    ZV_HIPCHECK(c, hipMemcpyAsync(d_spl, hsp.data(), sizeof(int) * hsp.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    // 3. nbl blocks of npts samples (the samples of a full zv_mc_blocks run
+    // with this seed and stream): densities of those shell pairs, the
+    // unit-pair estimator, statistics of the selected quartets
+    const bool single = precision != ZV_PREC_FP64;
+    const int Mc = pick_chunk(c, npts, precision);
+    const int np = c->npair, npp = c->npair_pad;
+    const size_t fbytes = single ? 4 : 8;
+    const size_t ncomp = (size_t)4 * Mc * np, ngrp = (size_t)6 * Mc * c->nsp;
+    ZV_CHECK(zv_ensure(c, c->b_r1, sizeof(double) * 3 * Mc));
+    ZV_CHECK(zv_ensure(c, c->b_r2, sizeof(double) * 3 * Mc));
+    ZV_CHECK(zv_ensure(c, c->b_pi0, sizeof(double) * Mc));
+    ZV_CHECK(zv_ensure(c, c->b_pd, fbytes * (ncomp + ngrp)));
+    ZV_CHECK(zv_ensure(c, c->b_accS, sizeof(double) * ntri));
+    ZV_CHECK(zv_ensure(c, c->b_s1, sizeof(double) * 2 * nsel));
+    double *accS = (double *)c->b_accS.p, *s1 = (double *)c->b_s1.p, *s2 = s1 + nsel;
+    double *d_r1 = (double *)c->b_r1.p, *d_r2 = (double *)c->b_r2.p, *d_pi0 = (double *)c->b_pi0.p;
+    const int *pos_pair = (const int *)c->b_gmeta.p;
+    const int *sp_first = pos_pair + c->grp_possp_off + np;
+    const Basis B = dev_basis(c);
+    const PairTab PT = dev_pairs(c);
     {
-        PhaseTimer tm(c, PH_COPY);
-        hipLaunchKernelGGL(k_merge_sel, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel,
-                           (const int64_t *)d_list, (const int64_t *)d_tri,
-                           (const double *)c->b_accS.p, (const double *)c->b_out2.p, res_mean,
-                           res_err);
+        PhaseTimer tm(c, PH_STATS);
+        hipLaunchKernelGGL(k_sel_zero, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel,
+                           (const int64_t *)d_tri, accS, s1, s2);
+        ZV_HIPCHECK(c, hipGetLastError());
+    }
+    for (int b = 0; b < nbl; b++) {
+        for (int64_t m0 = 0; m0 < npts; m0 += Mc) {
+            const int Mv = (int)((npts - m0) < Mc ? (npts - m0) : Mc);
+            const uint64_t g0 = (uint64_t)b * (uint64_t)npts + (uint64_t)m0;
+            {
+                PhaseTimer tm(c, PH_SAMPLE);
+                hipLaunchKernelGGL(k_draw, dim3(nblk(Mv, 256)), dim3(256), 0, c->stream, seed,
+                                   stream, g0, Mv, Mv, d_r1, d_r2, d_pi0);
+                ZV_HIPCHECK(c, hipGetLastError());
+            }
+            const dim3 dgrid(nblk(Mc, 256) * (unsigned)nspl);
+            if (single) {
+                float *comp = (float *)c->b_pd.p, *grp = comp + ncomp;
+                {
+                    PhaseTimer tm(c, PH_DENS);
+                    hipLaunchKernelGGL(k_dens_group<float>, dgrid, dim3(256), 0, c->stream, B, PT,
+                                       npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1, d_r2,
+                                       d_pi0, comp, grp, (const int *)d_spl);
+                    ZV_HIPCHECK(c, hipGetLastError());
+                }
+                PhaseTimer tm(c, PH_EST);
+                hipLaunchKernelGGL(k_est_pairs<float>, dim3(nup), dim3(256), 0, c->stream,
+                                   (const float *)comp, (const float *)grp, (const int *)d_up,
+                                   pos_pair, Mv, Mc, np, c->grp_nsp_pad, accS);
+                ZV_HIPCHECK(c, hipGetLastError());
+            } else {
+                double *comp = (double *)c->b_pd.p, *grp = comp + ncomp;
+                {
+                    PhaseTimer tm(c, PH_DENS);
+                    hipLaunchKernelGGL(k_dens_group<double>, dgrid, dim3(256), 0, c->stream, B,
+                                       PT, npp, pos_pair, sp_first, np, c->nsp, Mv, Mc, d_r1,
+                                       d_r2, d_pi0, comp, grp, (const int *)d_spl);
+                    ZV_HIPCHECK(c, hipGetLastError());
+                }
+                PhaseTimer tm(c, PH_EST);
+                hipLaunchKernelGGL(k_est_pairs<double>, dim3(nup), dim3(256), 0, c->stream,
+                                   (const double *)comp, (const double *)grp, (const int *)d_up,
+                                   pos_pair, Mv, Mc, np, c->grp_nsp_pad, accS);
+                ZV_HIPCHECK(c, hipGetLastError());
+            }
+        }
+        PhaseTimer tm(c, PH_STATS);
+        hipLaunchKernelGGL(k_sel_block_end, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel,
+                           (const int64_t *)d_tri, 1.0 / (double)npts, accS, s1, s2);
+        ZV_HIPCHECK(c, hipGetLastError());
+    }
+    {
+        PhaseTimer tm(c, PH_STATS);
+        hipLaunchKernelGGL(k_sel_final, dim3(nblk(nsel, 256)), dim3(256), 0, c->stream, nsel, nbl,
+                           (const double *)s1, (const double *)s2, (const int64_t *)d_list,
+                           res_mean, res_err);
         ZV_HIPCHECK(c, hipGetLastError());
     }
     return zv_run_result_max(c, ZV_RESULT_STDERR, max_err);

@@ -307,9 +307,11 @@ extern "C" int zv_set_tuning(zv_context *c, int key, int64_t value)
 {
     ZV_SERIALIZE;
     if (!c) return ZV_ERR_ARG;
-    if (key < 0 || key > ZV_TUNE_ERI_STREAMS)
+    if (key < 0 || key > ZV_TUNE_EST_TILES)
         return zv_fail(c, ZV_ERR_ARG, "unknown tuning key %d", key);
     if (value < 0) return zv_fail(c, ZV_ERR_ARG, "tuning value %lld < 0", (long long)value);
+    if (key == ZV_TUNE_EST_TILES && value > 1)
+        return zv_fail(c, ZV_ERR_ARG, "tile-shape set %lld: 0 or 1", (long long)value);
     c->tune[key] = value;
     return ZV_OK;
 }

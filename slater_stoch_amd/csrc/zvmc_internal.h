@@ -18,7 +18,7 @@
 #define ZV_LPAIR_MAX 8 // largest pair angular momentum supported at all
 #define ZV_SHMAX 15    // largest shell (Cartesian components sharing one Gaussian fit)
 #define ZV_MAXU 36     // components of the uniform side handled by one shell work item
-#define ZV_NSTAT 136   // ERI statistics slots (component and shell kernel variants)
+#define ZV_NSTAT 236   // ERI statistics slots (component and shell kernel variants)
 
 struct DevBuf {
     void *p = nullptr;
@@ -107,6 +107,7 @@ struct zv_context {
     int64_t grp_tile_off[9] = {};     // int2 offsets in b_gmeta
     int64_t grp_possp_off = 0;
     int grp_nsp_pad = 0;
+    int grp_tileset = 0;               // tile-shape set the tables were built for
     std::vector<int> h_gmeta;
     std::vector<int> h_pair_ucls, h_pair_uidx; // unit class and index of each AO pair
     // planned shell work of the two Schwarz phases (host copies kept alive for the copies)
@@ -122,7 +123,7 @@ struct zv_context {
     double t_ms[PH_N] = {};
     int64_t t_n[PH_N] = {};
     std::vector<int> pend_cls;        // ERI class of each pending timer (-1: none)
-    // ERI statistics: [0, 36) component kernels 6 LB + LK; [36, 136) shell
+    // ERI statistics: [0, 36) component kernels 6 LB + LK; [36, 236) shell
     // kernels 36 + 25 v + 5 LU + LL, v = 2 (lane batch - 1) + reduction mode
     double cls_ms[ZV_NSTAT] = {};
     int64_t cls_n[ZV_NSTAT] = {};

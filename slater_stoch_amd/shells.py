@@ -76,3 +76,29 @@ def kernel_unit_pairs(bas):
         lo = np.array([u.min() for u in units])
         total += float(np.sum(hi[:, None] >= lo[None, :]))
     return total
+
+
+def mfma_tile_use(bas, tile=16):
+    """How well the estimator's per-shell-pair-pair products would fill
+    matrix-core tiles.  For a row shell pair s and a column shell pair t the
+    estimator's sum over samples, sum_m K_m(s,t) A_P,m B_Q,m for P in s and
+    Q in t, is an |s| x |t| product with the samples as the inner dimension;
+    on MFMA it runs as ceil(|s|/tile) x ceil(|t|/tile) tiles of tile x tile.
+    Returns (share of the canonical quartets whose shell pairs both have at
+    least ``tile`` components, useful fraction of the tile area summed over
+    all shell-pair pairs weighted by their quartets)."""
+    first = shell_firsts(bas)
+    sizes = []
+    for I in range(len(first) - 1):
+        for K in range(I + 1):
+            ni, nk = first[I + 1] - first[I], first[K + 1] - first[K]
+            sizes.append(ni * (ni + 1) // 2 if I == K else ni * nk)
+    w = np.array(sizes, dtype=np.float64)
+    pad = np.ceil(w / tile) * tile
+    big = w >= tile
+    # all ordered (s, t) pairs: quartets ~ |s||t| (the canonical half is the
+    # same share for every class up to the diagonal)
+    quart = np.outer(w, w)
+    share = float(quart[np.ix_(big, big)].sum() / quart.sum())
+    used = float(quart.sum() / np.outer(pad, pad).sum())
+    return share, used

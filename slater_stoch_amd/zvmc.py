@@ -59,7 +59,7 @@ SIGNATURES = {
 
 NOALLOC_MAXQ = 65536
 
-TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS, TUNE_ERI_STREAMS = 0, 1, 2, 3
+TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS, TUNE_ERI_STREAMS, TUNE_EST_TILES = 0, 1, 2, 3, 4
 
 RESULT_ERI, RESULT_MEAN, RESULT_STDERR = 0, 1, 2
 
@@ -144,8 +144,8 @@ class Engine:
 
     def set_tuning(self, key, value):
         """Tuning knob (TUNE_EST_SPLIT, TUNE_MC_CHUNK, TUNE_ERI_ITEMS,
-        TUNE_ERI_STREAMS); 0
-        restores the library's automatic choice."""
+        TUNE_ERI_STREAMS, TUNE_EST_TILES); 0 restores the library's
+        automatic choice."""
         self._check(self.lib.zv_set_tuning(self.h, int(key), int(value)))
 
     # ------------------------------------------------------------ basis
@@ -327,7 +327,7 @@ class Engine:
         """Per ERI kernel of the last gauss_eri / gauss_ijkl: {key: dict(ms,
         launches, units, flops)} with key ("component", LB, LK) or ("shell",
         LU, LL, lane_batch, reduction)."""
-        n_stat = 136
+        n_stat = 236
         ms, un, fl = np.zeros(n_stat), np.zeros(n_stat), np.zeros(n_stat)
         n = np.zeros(n_stat, np.int64)
         self._check(self.lib.zv_last_eri_stats(self.h, _p(ms), _p(n), _p(un), _p(fl)))

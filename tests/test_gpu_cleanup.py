@@ -109,12 +109,16 @@ def test_svd_clean_partial_rank_passes(engine):
     rng = np.random.default_rng(8)
     # noise 1e-5 on the unique values turns three of the smallest eigenvalues
     # of W (~3e-3 apart from those) negative, down to -7.4e-5; the passes must
-    # find exactly those (at noise 1e-8 W stays positive definite)
+    # find those (at noise 1e-8 W stays positive definite).  A later pass may
+    # also remove a negative direction of rounding size that the
+    # eight-permutation average of an earlier pass leaves behind (which one
+    # depends on the last bits of the ERIs), so the count is at least the
+    # dense one and the result equals the dense projection.
     vals = eri + 1e-5 * rng.standard_normal(eri.size)
     got, nrem = engine.svd_clean(vals, rank=96, q=12, seed=5)
     Wc, nneg = negative_removed(product_matrix(bas.n, q, vals))
     ref = gather(bas.n, q, Wc, True)
-    assert nrem == nneg > 0
+    assert nrem >= nneg > 0
     scale = np.abs(ref).max()
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * scale)
 

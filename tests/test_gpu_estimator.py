@@ -81,19 +81,24 @@ def stratified_quartets(bas, per_class, seed):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("split", [1, 4])
-def test_split_and_unsplit_instances_small_bases(engine, oracle, prec, split):
-    from slater_stoch_amd.zvmc import TUNE_EST_SPLIT
+@pytest.mark.parametrize("tiles", [0, 1])
+def test_split_and_unsplit_instances_small_bases(engine, oracle, prec, split, tiles):
+    """Both tile-shape sets (ZV_TUNE_EST_TILES), each split and unsplit."""
+    from slater_stoch_amd.zvmc import TUNE_EST_SPLIT, TUNE_EST_TILES
     try:
         engine.set_tuning(TUNE_EST_SPLIT, split)
+        engine.set_tuning(TUNE_EST_TILES, tiles)
         for bas, nbl, npts in ((configs.water_minimal(ng0=4), 4, 400), (_mixed_basis(), 2, 40)):
             engine.set_basis(bas)
             oracle.set_basis(bas)
             r1, r2, pi0 = _samples(oracle, nbl * npts, 3)
             ref, scale, scale2 = fp64_estimate(oracle, bas, r1, r2, pi0, nbl, npts, split=True)
             got, _ = engine.mc_blocks(nbl, npts, prec, samples=(r1, r2, pi0))
-            _check(got, ref, scale2 if prec == "bf16" else scale, prec, (prec, split, bas.n))
+            _check(got, ref, scale2 if prec == "bf16" else scale, prec,
+                   (prec, split, tiles, bas.n))
     finally:
         engine.set_tuning(TUNE_EST_SPLIT, 0)
+        engine.set_tuning(TUNE_EST_TILES, 0)
 
 
 @pytest.fixture(scope="module")

@@ -1,10 +1,12 @@
 """Targeted refinement (zv_mc_refine): quartets whose kept standard error is
-above the target get a fresh run restricted to the estimator tiles that hold
-them.  With the sample split fixed (ZV_TUNE_EST_SPLIT = 1) a tile sums its
-samples in the same order whichever tiles run, so
+above the target get a fresh run of the unit-pair estimator over the unit
+pairs that hold them, on the samples a full zv_mc_blocks run with the same
+seed, stream and blocks would draw, so
 
-* every refined quartet equals, bit for bit, a full zv_mc_blocks run with
-  the same samples (same seed, stream, blocks and precision);
+* every refined quartet equals that full run's value up to summation order:
+  within 1e-3 of its standard error (reordering a sum of M terms changes it
+  by ~eps sum|t|, about eps sqrt(M) standard errors: 1e-14 in FP64, 6e-6 in
+  FP32 at M = 8000);
 * every other quartet keeps its pilot mean and standard error, bit for bit;
 * the count of refined quartets and the reported maximum match the arrays.
 """
@@ -43,8 +45,8 @@ def test_refine_equals_full_run_on_selected_quartets(engine, prec, name):
     finally:
         engine.set_tuning(Z.TUNE_EST_SPLIT, 0)
     assert nref == int(sel.sum()) > 0
-    np.testing.assert_array_equal(got_m[sel], fm[sel])
-    np.testing.assert_array_equal(got_e[sel], fe[sel])
+    np.testing.assert_array_less(np.abs(got_m[sel] - fm[sel]), 1e-3 * fe[sel] + 1e-300)
+    np.testing.assert_array_less(np.abs(got_e[sel] - fe[sel]), 1e-3 * fe[sel] + 1e-300)
     np.testing.assert_array_equal(got_m[~sel], pm[~sel])
     np.testing.assert_array_equal(got_e[~sel], pe[~sel])
     assert emax == got_e.max()
