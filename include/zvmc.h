@@ -230,16 +230,26 @@ int zv_rsvd_clean(zv_context *ctx, int rank, int q, uint64_t seed, int sketch, d
 
 /* Block Davidson for the nev (1..64) lowest eigenpairs of the same V (the
  * role of src/davidson.F90:1-159 in the iterative branch of
- * src/davidson_clean.F90:120-323): Jacobi-preconditioned corrections, an
- * orthonormal search space of up to max(8 nev, 48) vectors with thick
- * restart, FP64 throughout.  Stops when every residual norm
- * ||V x - theta x|| <= tol (Hartree) or after max_iter iterations; the
- * negative eigenpairs whose residual is within tol are removed from V
- * (eri in/out).  evals (out, nev, may be NULL): the Ritz values ascending;
- * resid (out, nev, may be NULL): their residual norms; n_removed, iters
- * (out, may be NULL). */
-int zv_davidson_lowest(zv_context *ctx, int nev, double tol, int max_iter, double *eri,
-                       double *evals, double *resid, int *n_removed, int *iters);
+ * src/davidson_clean.F90:120-323): the residuals of the unconverged Ritz
+ * vectors as corrections (no diagonal preconditioner: diag V spans orders of
+ * magnitude while the wanted eigenvalues lie near zero), an orthonormal
+ * search space of up to max(8 nev, 48) vectors with thick restart, FP64
+ * throughout.  precond ZV_DAVIDSON_SHIFT_INVERT replaces the residual by
+ * (V - sigma I)^-1 r, with the Cholesky factor of V - sigma I for a shift
+ * sigma below the lowest eigenvalue (the factorization succeeding certifies
+ * it; one extra N x N matrix): after 8 plain iterations sigma = theta_1 -
+ * (2 ||r_1|| + 1e-3 |theta_1|), lowered fourfold on each failed
+ * factorization and moved up to theta_1 - 2 ||r_1|| whenever the residuals
+ * have dropped a hundredfold (at most 4 factorizations).  Stops when every
+ * residual norm ||V x - theta x|| <= tol (Hartree) or after max_iter
+ * iterations; the negative eigenpairs whose residual is within tol are
+ * removed from V (eri in/out).  evals (out, nev, may be NULL): the Ritz
+ * values ascending; resid (out, nev, may be NULL): their residual norms;
+ * n_removed, iters (out, may be NULL). */
+#define ZV_DAVIDSON_PLAIN 0
+#define ZV_DAVIDSON_SHIFT_INVERT 1
+int zv_davidson_lowest(zv_context *ctx, int nev, double tol, int max_iter, int precond,
+                       double *eri, double *evals, double *resid, int *n_removed, int *iters);
 
 /* Progress lines of the long-running calls, written to stdout: level 1
  * prints what the reference prints unconditionally in svd_clean ("Vectors

@@ -15,9 +15,9 @@ are this package's:
   nw = 40 and capped by what is left of max_samples (the estimator's error is
   heavy-tailed, so one run sized from the pilot can fall short).  With the
   default policy "refine" that run covers only the quartets still above the
-  target (zv_mc_refine: the estimator tiles that hold them), which replace
-  their values with the new run's; with "rerun" it covers every quartet
-  (the earlier policy, for comparison);
+  target (zv_mc_refine: the estimator unit pairs that hold them), which
+  replace their values with the new run's; with "rerun" it covers every
+  quartet (the earlier policy, for comparison);
 * every reported mean and standard error is one complete nbl-block estimate
   of a single run, as two_e_slater writes it.  ``samples`` counts the
   samples of every run (a refinement run's samples serve its quartets only).
@@ -31,6 +31,13 @@ streams and HIP opens four hardware queues per process by default, so
 unrelated streams waited behind each other's long estimator launches), and
 a run with three workers hung.  The library runs calls on different
 contexts one at a time, and ``workers`` must be 1.
+
+Each molecule's result also carries the device time of its library calls
+(the library's own event timers, all phases) and their kernel launches:
+the wall time not spent on the device is the host's share (planning,
+launch and synchronization), the most that capturing the launches in a
+graph could save.  ``--autotune`` first tunes the launch knobs per molecule
+size class (slater_stoch_amd/autotune.py).
 
 Command line (the 64-molecule batch of uneven sizes, FP32 estimator):
 
@@ -67,12 +74,21 @@ def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=400
     from .zvmc import RESULT_ERI, RESULT_MEAN, RESULT_STDERR
     if policy not in ("refine", "rerun"):
         raise ValueError(f"unknown policy {policy!r}")
+    dev = [0.0, 0]  # device time (s) and kernel launches of the library's calls
+
+    def count():
+        for ms, n in eng.timings().values():
+            dev[0] += ms / 1e3
+            dev[1] += int(n)
+
     t0 = time.perf_counter()
     eng.set_basis(bas)
     _, _, nscr = eng.gauss_eri(device_only=True)
+    count()
     t1 = time.perf_counter()
     npts, rounds, used, refined = pilot_npts, 0, 0, 0
     eng.mc_blocks(nbl, npts, precision, seed=seed, stream=0, device_only=True)
+    count()
     emax = eng.result_max(RESULT_STDERR)
     used += nbl * npts
     rounds += 1
@@ -84,16 +100,19 @@ def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=400
         if policy == "refine":
             nref, emax = eng.mc_refine(nbl, npts, precision, target=target, seed=seed,
                                        stream=rounds)
+            count()
             refined += nref
         else:
             eng.mc_blocks(nbl, npts, precision, seed=seed, stream=rounds, device_only=True)
+            count()
             emax = eng.result_max(RESULT_STDERR)
         used += nbl * npts
         rounds += 1
     t2 = time.perf_counter()
     out = dict(n=int(bas.n), nint=int(bas.nint), n_screened=int(nscr), samples=int(used),
                rounds=rounds, npts=int(npts), max_stderr=emax, reached=bool(emax <= target),
-               refined_quartets=int(refined), t_analytic=t1 - t0, t_mc=t2 - t1)
+               refined_quartets=int(refined), t_analytic=t1 - t0, t_mc=t2 - t1,
+               t_device=dev[0], launches=dev[1])
     if keep_arrays:
         eri = eng.get_result(RESULT_ERI)
         mean, err = eng.get_result(RESULT_MEAN), eng.get_result(RESULT_STDERR)
@@ -148,7 +167,9 @@ def summary(results, wall):
                 quartet_samples=float(sum(float(r["nint"]) * r["samples"] for r in results)),
                 tune_s=float(sum(r.get("t_tune", 0.0) for r in results)),
                 analytic_s=float(sum(r["t_analytic"] for r in results)),
-                mc_s=float(sum(r["t_mc"] for r in results)))
+                mc_s=float(sum(r["t_mc"] for r in results)),
+                device_s=float(sum(r.get("t_device", 0.0) for r in results)),
+                launches=int(sum(r.get("launches", 0) for r in results)))
 
 
 def main(argv=None):

@@ -590,7 +590,9 @@ int zv_run_davidson_clean(zv_context *c, double *eri, int *n_removed)
 // while diag V = (P|P) spans orders of magnitude, so R / (diag V - theta)
 // damps exactly the components the wanted vectors carry (on a noisy 30-AO
 // matrix it had not converged after 2000 iterations; without it, 167).  The
-// converged negative pairs are removed from V.
+// optional shift-invert preconditioner (V - sigma I)^-1, with sigma below
+// the spectrum, serves instead (shift_factor).  The converged negative pairs
+// are removed from V.
 namespace {
 
 // V from the quartet list: the kcp entry of (P, Q) at V[P,Q] and V[Q,P]
@@ -678,6 +680,14 @@ __global__ void k_mirror(int m, int ld, double *H)
     if (t >= (int64_t)m * m) return;
     const int i = (int)(t % m), j = (int)(t / m);
     if (i > j) H[i + (int64_t)j * ld] = H[j + (int64_t)i * ld];
+}
+
+// F = V - sigma I
+__global__ void k_shifted_copy(int64_t N, const double *V, double sigma, double *F)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * N) return;
+    F[t] = V[t] - ((t % N) == (t / N) ? sigma : 0.0);
 }
 
 } // namespace
@@ -863,8 +873,48 @@ int zv_run_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int sketch3
     return gather_V(c, V, d_eri, eri, N);
 }
 
-int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
-                           double *evals, double *resid, int *n_removed, int *iters)
+// Shift-invert preconditioner for the lowest eigenpairs: the Cholesky
+// factor of V - sigma I with sigma below the lowest eigenvalue (the
+// factorization succeeds exactly when V - sigma I is positive definite,
+// which certifies sigma < lambda_min), so that a correction
+// (V - sigma I)^-1 r = x + (sigma - theta)(V - sigma I)^-1 x is an
+// inverse-iteration step toward the eigenvalues just above sigma.  sigma
+// starts at theta_1 - (2 ||r_1|| + 1e-3 |theta_1|) and steps down four
+// times further on each failed factorization (at most 12); once the
+// residuals have dropped a hundredfold it is moved up to theta_1 -
+// 2 ||r_1|| (at most 4 factorizations).  Returns false when no shift
+// factorized (the iteration then goes on unpreconditioned).
+static int shift_factor(zv_context *c, rocblas_handle h, const double *V, int64_t N,
+                        double theta1, double gap, double *F, rocblas_int *info, double *sigma,
+                        bool *ok)
+{
+    *ok = false;
+    for (int tries = 0; tries < 12; tries++, gap *= 4.0) {
+        if (tries > 0 && gap == 0.0) break;
+        const double sg = theta1 - gap;
+        hipLaunchKernelGGL(k_shifted_copy, dim3(nblk(N * N, 256)), dim3(256), 0, c->stream, N, V,
+                           sg, F);
+        ZV_HIPCHECK(c, hipGetLastError());
+        RB(c, rocsolver_dpotrf(h, rocblas_fill_upper, (rocblas_int)N, F, (rocblas_int)N, info));
+        rocblas_int hinfo = 0;
+        ZV_HIPCHECK(c, hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                      c->stream));
+        ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (c->verbose > 1) {
+            printf(" davidson: shift %.6e %s\n", sg, hinfo == 0 ? "factorized" : "not definite");
+            fflush(stdout);
+        }
+        if (hinfo == 0) {
+            *sigma = sg;
+            *ok = true;
+            return ZV_OK;
+        }
+    }
+    return ZV_OK;
+}
+
+int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, int precond,
+                           double *eri, double *evals, double *resid, int *n_removed, int *iters)
 {
     const int64_t N = c->npair;
     const int k = nev;
@@ -881,7 +931,7 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
     // V, S and AS (N mmax), X, AX, R, T, AT, bk (N k), H, Hw (mmax^2), C
     // (mmax k), theta, E, tau, norms (mmax), G (k^2), the quartet list, info
     const size_t words = nn + 2 * nm + 6 * nk + 2 * mm + (size_t)mmax * k + 4 * (size_t)mmax +
-                         (size_t)k * k + (size_t)c->nint + 16;
+                         (size_t)k * k + (size_t)c->nint + 16 + (precond ? nn : 0);
     ZV_CHECK(zv_ensure(c, c->b_misc, sizeof(double) * words));
     ZV_CHECK(zv_ensure(c, c->b_part, sizeof(double) * (kGramSplit + 1) * (size_t)k * k));
     double *V = (double *)c->b_misc.p, *S = V + nn, *AS = S + nm, *X = AS + nm, *AX = X + nk,
@@ -889,6 +939,7 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
            *Hw = H + mm, *C = Hw + mm, *theta = C + (size_t)mmax * k, *E = theta + mmax,
            *tau = E + mmax, *nrm = tau + mmax, *G = nrm + mmax, *d_eri = G + (size_t)k * k;
     rocblas_int *info = (rocblas_int *)(d_eri + c->nint);
+    double *F = precond ? d_eri + c->nint + 16 : nullptr; // Cholesky factor of V - sigma I
     double *part = (double *)c->b_part.p;
     const OrthoWork ow{G, bk, tau, info + 1, part};
     int n_fallback = 0;
@@ -908,6 +959,9 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
     int m = k, it = 0;
     std::vector<double> htheta(mmax), hnrm(k);
     bool converged = false;
+    bool have_pc = false;
+    int nfact = 0;
+    double sigma = 0.0, r_at_fact = 0.0;
     for (;;) {
         // Ritz pairs of the space: Hw = H(0:m, 0:m) (upper triangle valid)
         hipLaunchKernelGGL(k_mirror, dim3(nblk((int64_t)m * m, 256)), dim3(256), 0, c->stream, m,
@@ -949,6 +1003,28 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
         }
         if (it >= max_iter) break;
         it++;
+        // shift-invert preconditioner: first after 8 plain iterations (the
+        // Ritz values have left the bulk of the spectrum), then moved up
+        // toward theta_1 each time the residuals have dropped a hundredfold
+        if (precond && nfact < 4 && it >= 8 && (!have_pc || rmax < 0.01 * r_at_fact)) {
+            const double gap = have_pc ? 2.0 * hnrm[0] + 1e-300
+                                       : 2.0 * hnrm[0] + 1e-3 * std::fabs(htheta[0]) + 1e-300;
+            bool ok = false;
+            double sg = 0.0;
+            ZV_CHECK(shift_factor(c, h, V, N, htheta[0], gap, F, info, &sg, &ok));
+            nfact++;
+            if (ok) {
+                have_pc = true;
+                sigma = sg;
+                r_at_fact = rmax;
+            } else if (have_pc) {
+                // the factor of the previous shift was overwritten: refactor it
+                bool ok2 = false;
+                ZV_CHECK(shift_factor(c, h, V, N, sigma, 0.0, F, info, &sg, &ok2));
+                have_pc = ok2;
+                r_at_fact = rmax;
+            }
+        }
         // the residuals of the unconverged Ritz vectors extend the space
         int nt = 0;
         for (int j = 0; j < k; j++)
@@ -958,6 +1034,9 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
                                               c->stream));
                 nt++;
             }
+        if (have_pc) // T <- (V - sigma I)^-1 T
+            RB(c, rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N, nt, F, (rocblas_int)N, T,
+                                   (rocblas_int)N));
         if (m + nt > mmax) {
             // thick restart: the Ritz vectors span the new space (orthonormal)
             ZV_HIPCHECK(c, hipMemcpyAsync(S, X, sizeof(double) * nk, hipMemcpyDeviceToDevice,
@@ -1013,8 +1092,9 @@ int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, dou
     }
     if (c->verbose) {
         printf(" davidson: %d lowest eigenvalues of the pair supermatrix, %s after %d "
-               "iterations, lowest %g, %d negative removed\n",
-               k, converged ? "converged" : "not all converged", it, htheta[0], neg);
+               "iterations, lowest %g, %d negative removed%s\n",
+               k, converged ? "converged" : "not all converged", it, htheta[0], neg,
+               have_pc ? ", shift-invert preconditioned" : "");
         fflush(stdout);
     }
     if (evals) std::copy(htheta.begin(), htheta.begin() + k, evals);

@@ -309,7 +309,11 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 for (int j = 0; j < CLB; j++)
 #pragma unroll
                     for (int h = 0; h < HL; h++) {
-                        const double e = pp.E[el[j] + (int64_t)h * capl[j] + cc];
+                        // (cl + j < it.cl1 always holds; as a condition it keeps
+                        // the compiler from issuing these loads one at a time
+                        // next to their uses, which serialized their latency)
+                        const double e =
+                            cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
                         El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -(e * f) : e * f;
                     }
 #pragma unroll

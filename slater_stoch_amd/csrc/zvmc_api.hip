@@ -749,8 +749,9 @@ extern "C" int zv_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int 
     return rc;
 }
 
-extern "C" int zv_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
-                                  double *evals, double *resid, int *n_removed, int *iters)
+extern "C" int zv_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, int precond,
+                                  double *eri, double *evals, double *resid, int *n_removed,
+                                  int *iters)
 {
     ZV_SERIALIZE;
     ZV_NEED_BASIS(c);
@@ -758,7 +759,10 @@ extern "C" int zv_davidson_lowest(zv_context *c, int nev, double tol, int max_it
     if (nev < 1 || nev > 64) return zv_fail(c, ZV_ERR_ARG, "nev %d outside 1..64", nev);
     if (!(tol > 0.0) || max_iter < 1)
         return zv_fail(c, ZV_ERR_ARG, "tol > 0 and max_iter >= 1 required");
-    int rc = zv_run_davidson_lowest(c, nev, tol, max_iter, eri, evals, resid, n_removed, iters);
+    if (precond != ZV_DAVIDSON_PLAIN && precond != ZV_DAVIDSON_SHIFT_INVERT)
+        return zv_fail(c, ZV_ERR_ARG, "unknown Davidson preconditioner %d", precond);
+    int rc = zv_run_davidson_lowest(c, nev, tol, max_iter, precond, eri, evals, resid, n_removed,
+                                    iters);
     if (rc == ZV_OK) rc = zv_collect_timings(c);
     return rc;
 }

@@ -198,8 +198,8 @@ int zv_run_mc_refine(zv_context *c, int nbl, int64_t npts, uint64_t seed, uint64
                      int precision, double target, int64_t *n_refined, double *max_err);
 int zv_run_rsvd_clean(zv_context *c, int rank, int q, uint64_t seed, int sketch32, double *eri,
                       double *ritz, int *n_removed);
-int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, double *eri,
-                           double *evals, double *resid, int *n_removed, int *iters);
+int zv_run_davidson_lowest(zv_context *c, int nev, double tol, int max_iter, int precond,
+                           double *eri, double *evals, double *resid, int *n_removed, int *iters);
 
 #define ZV_CHECK(call)                                                                            \
     do {                                                                                          \

@@ -53,7 +53,7 @@ SIGNATURES = {
     "zv_set_tuning": (I32, [VP, I32, I64]),
     "zv_mc_refine": (I32, [VP, I32, I64, U64, U64, I32, F64, VP, VP]),
     "zv_rsvd_clean": (I32, [VP, I32, I32, U64, I32, VP, VP, VP]),
-    "zv_davidson_lowest": (I32, [VP, I32, F64, I32, VP, VP, VP, VP, VP]),
+    "zv_davidson_lowest": (I32, [VP, I32, F64, I32, I32, VP, VP, VP, VP, VP]),
     "zv_gauss_ijkl_list_noalloc": (I32, [VP, I64, VP, VP, VP, VP, VP]),
 }
 
@@ -311,15 +311,17 @@ class Engine:
                                            _p(nrem)))
         return e, ritz, int(nrem[0])
 
-    def davidson_lowest(self, eri, nev=10, tol=1e-9, max_iter=1000):
+    def davidson_lowest(self, eri, nev=10, tol=1e-9, max_iter=1000, precond="none"):
         """Block Davidson on the pair supermatrix: (cleaned eri, the nev
         lowest Ritz values, their residual norms, number removed,
-        iterations)."""
+        iterations).  precond "shift-invert" uses the Cholesky factor of
+        V - sigma I below the spectrum (one more N x N matrix)."""
+        code = {"none": 0, "shift-invert": 1}[precond]
         e = np.array(eri, np.float64, copy=True)
         ev, res = np.zeros(int(nev)), np.zeros(int(nev))
         nrem = np.zeros(1, np.int32)
         its = np.zeros(1, np.int32)
-        self._check(self.lib.zv_davidson_lowest(self.h, int(nev), float(tol), int(max_iter),
+        self._check(self.lib.zv_davidson_lowest(self.h, int(nev), float(tol), int(max_iter), code,
                                                 _p(e), _p(ev), _p(res), _p(nrem), _p(its)))
         return e, ev, res, int(nrem[0]), int(its[0])
 

@@ -58,6 +58,9 @@ class FakeEngine:
     def result_max(self, which):
         return float(self.err.max())
 
+    def timings(self):
+        return {"estimator": (1.0, 9)}
+
 
 class FakeBasis:
     n, nint = 3, 5
@@ -73,6 +76,8 @@ def test_policy_reaches_target_in_two_runs():
     assert eng.calls[1][1] == 19200
     assert r["samples"] == 10 * 4000 + 10 * 19200
     assert r["max_stderr"] == pytest.approx(2e-6 * np.sqrt(40000 / 192000))
+    # device time and launches summed over the three library calls
+    assert r["t_device"] == pytest.approx(0.003) and r["launches"] == 27
 
 
 def test_policy_stops_at_the_sample_budget():
