@@ -63,7 +63,7 @@ def _measure(eng, bas, settings, npts, samples_per_job, precision, reps):
     return a + e * samples_per_job, a, e
 
 
-def tune(eng, bas, precision="fp32", npts=800, samples_per_job=40000, reps=2, knobs=None):
+def tune(eng, bas, precision="fp32", npts=4000, samples_per_job=40000, reps=1, knobs=None):
     """Best settings for this molecule: (settings dict {key: value}, log of
     every trial).  The engine's settings are left at the best ones."""
     knobs = KNOBS if knobs is None else knobs
@@ -122,3 +122,10 @@ def apply(eng, settings):
 def to_json(table):
     return json.dumps({str(k): {NAMES[key]: v for key, v in s.items()}
                        for k, s in table.items()})
+
+
+def from_json(text):
+    """The table written by to_json."""
+    keys = {v: k for k, v in NAMES.items()}
+    return {int(c): {keys[name]: int(v) for name, v in s.items()}
+            for c, s in json.loads(text).items()}

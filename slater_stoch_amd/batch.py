@@ -121,7 +121,7 @@ def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=400
 
 
 def run_batch(bases, target=1e-6, precision="fp32", workers=1, device=0, seed=0,
-              callback=None, verbose=0, tuning=None, autotune=False, **policy):
+              callback=None, verbose=0, tuning=None, autotune=False, on_tuned=None, **policy):
     """Run every basis of ``bases``; returns the per-molecule dicts in input
     order.  ``callback(index, result)`` is called as each molecule finishes.
     ``verbose`` is the engine's progress level (2: a line per finished Monte
@@ -132,7 +132,7 @@ def run_batch(bases, target=1e-6, precision="fp32", workers=1, device=0, seed=0,
     ``tuning`` is a table {size class: {knob: value}} (autotune.tune_classes)
     applied before each molecule of that class; ``autotune=True`` builds it
     first on this batch, its time counted in the results' "t_tune" of the
-    first molecule run."""
+    first molecule run, and passes it to ``on_tuned``."""
     if int(workers) != 1:
         raise ValueError("run_batch: workers must be 1 (contexts driven from several host "
                          "threads at once are not supported)")
@@ -147,6 +147,8 @@ def run_batch(bases, target=1e-6, precision="fp32", workers=1, device=0, seed=0,
             t0 = time.perf_counter()
             tuning, _ = AT.tune_classes(eng, bases, precision)
             t_tune = time.perf_counter() - t0
+            if on_tuned is not None:
+                on_tuned(tuning)
         for t in order:
             AT.apply(eng, AT.settings_for(tuning, bases[t]))
             r = run_molecule(eng, bases[t], target=target, precision=precision, seed=seed + t,
@@ -188,6 +190,8 @@ def main(argv=None):
     ap.add_argument("--policy", default="refine", choices=["refine", "rerun"])
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch parameters per size class first (timed in wall_s)")
+    ap.add_argument("--tuning-file", default=None,
+                    help="with --autotune: write the tuned table here; without: read it")
     ap.add_argument("--verbose", type=int, default=2,
                     help="engine progress level (2: one line per Monte Carlo block)")
     args = ap.parse_args(argv)
@@ -198,13 +202,28 @@ def main(argv=None):
               f"{r['t_analytic']:.2f} s, MC {r['t_mc']:.2f} s, {r['samples']} samples in "
               f"{r['rounds']} runs, max stderr {r['max_stderr']:.2e}", flush=True)
 
+    from . import autotune as AT
+    tuning = None
+    if args.tuning_file and not args.autotune:
+        with open(args.tuning_file) as f:
+            tuning = AT.from_json(f.read())
+    saved = {}
+
+    def keep_table(table):
+        saved["table"] = table
+
     t0 = time.perf_counter()
     res = run_batch(bases, target=args.target, precision=args.precision, workers=args.workers,
                     callback=show, verbose=args.verbose, max_samples=args.max_samples,
-                    max_rounds=args.max_rounds, policy=args.policy, autotune=args.autotune)
+                    max_rounds=args.max_rounds, policy=args.policy, autotune=args.autotune,
+                    tuning=tuning, on_tuned=keep_table)
+    if args.autotune and args.tuning_file and "table" in saved:
+        with open(args.tuning_file, "w") as f:
+            f.write(AT.to_json(saved["table"]))
     s = summary(res, time.perf_counter() - t0)
     s.update(precision=args.precision, workers=args.workers, target=args.target,
-             policy=args.policy, autotune=args.autotune)
+             policy=args.policy, autotune=args.autotune,
+             tuning=AT.to_json(tuning) if tuning else None)
     print(json.dumps(s), flush=True)
     return 0
 

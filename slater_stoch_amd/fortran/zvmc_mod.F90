@@ -13,6 +13,9 @@ module zvmc
   integer(c_int), parameter :: ZV_SAMPLES_FIXED = 0, ZV_SAMPLES_PHILOX = 1
   integer(c_int), parameter :: ZV_PREC_FP64_PARITY = 0, ZV_PREC_FP64 = 1, ZV_PREC_FP32 = 2
   integer(c_int), parameter :: ZV_PREC_BF16 = 3
+  integer(c_int), parameter :: ZV_RESULT_ERI = 0, ZV_RESULT_MEAN = 1, ZV_RESULT_STDERR = 2
+  integer(c_int), parameter :: ZV_SKETCH_FP64 = 0, ZV_SKETCH_FP32 = 1
+  integer(c_int), parameter :: ZV_DAVIDSON_PLAIN = 0, ZV_DAVIDSON_SHIFT_INVERT = 1
 
   interface
     integer(c_int) function zv_create(device, ctx) bind(C, name="zv_create")
@@ -136,6 +139,36 @@ module zvmc
       type(c_ptr), value :: ctx
       integer(c_int), value :: key
       integer(c_int64_t), value :: val
+    end function
+    integer(c_int) function zv_mc_refine(ctx, nbl, npts, seed, stream, precision, target, &
+        n_refined, max_err) bind(C, name="zv_mc_refine")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbl, precision
+      integer(c_int64_t), value :: npts, seed, stream
+      real(c_double), value :: target
+      integer(c_int64_t), intent(out) :: n_refined
+      real(c_double), intent(out) :: max_err
+    end function
+    integer(c_int) function zv_rsvd_clean(ctx, rank, q, seed, sketch, eri, ritz, n_removed) &
+        bind(C, name="zv_rsvd_clean")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: rank, q, sketch
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(inout) :: eri(*)
+      type(c_ptr), value :: ritz
+      integer(c_int), intent(out) :: n_removed
+    end function
+    integer(c_int) function zv_davidson_lowest(ctx, nev, tol, max_iter, precond, eri, evals, &
+        resid, n_removed, iters) bind(C, name="zv_davidson_lowest")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nev, max_iter, precond
+      real(c_double), value :: tol
+      real(c_double), intent(inout) :: eri(*)
+      real(c_double), intent(out) :: evals(*), resid(*)
+      integer(c_int), intent(out) :: n_removed, iters
     end function
   end interface
 

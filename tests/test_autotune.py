@@ -63,6 +63,7 @@ def test_size_classes_tune_the_largest_member_once():
     assert sorted(table) == [AT.size_class(Bas(18)), AT.size_class(Bas(35)),
                              AT.size_class(Bas(150))]
     assert json.loads(AT.to_json(table))[str(AT.size_class(Bas(35)))]["est_split"] == 4
+    assert AT.from_json(AT.to_json(table)) == table
 
 
 def test_settings_for_falls_back_to_the_nearest_tuned_class():
