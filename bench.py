@@ -261,9 +261,10 @@ def eri_by_class(res):
 
 def sample_stream(eng, bas, npts):
     """The configuration's Monte Carlo stream (1e8 samples in 1e6-sample
-    chunks, every quartet) as a throughput: one run of 10 blocks of npts
-    samples per precision, quartet-samples per second and the time 1e8
-    samples would take at that rate."""
+    chunks, every quartet) as a throughput: one chunk (10 blocks of npts
+    samples; 1e6 samples by default) per precision, quartet-samples per
+    second and the time the hundred chunks of 1e8 samples would take at
+    that rate."""
     eng.set_basis(bas)
     out = {}
     for prec in ("fp64", "fp32"):
@@ -419,8 +420,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only: skip the FP32 job, the 110-AO and water reports, "
                          "the sample stream and the cleanups")
-    ap.add_argument("--stream-npts", type=int, default=10000,
-                    help="samples per block of the sample-stream throughput run")
+    ap.add_argument("--stream-npts", type=int, default=100000,
+                    help="samples per block of the sample-stream throughput run (10 blocks: "
+                         "the default is one 1e6-sample chunk)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
