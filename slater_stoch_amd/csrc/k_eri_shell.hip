@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
 // Direct mode (few uniform components): every lane contracts its X with the
 // E of every uniform component, scalar E operands, nu * HU FMAs per lane and
 // primitive quartet; per-lane sums over the uniform components live in LDS.
-template <int LU, int LL, int CLB>
+template <int LU, int LL, int CLB, bool PFU>
 __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                                                   const ShellItem *__restrict__ items,
                                                   const double *__restrict__ pu_all,
@@ -124,6 +124,8 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
     __shared__ double accs[kDirectMaxU * CLB * 64]; // it.nu <= kDirectMaxU (host-checked)
     const int lane = threadIdx.x;
+    int vzero = 0;
+    if constexpr (PFU) asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
     const ShellItem it = items[blockIdx.x];
     const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
     const int ncu = sd.coff[it.su + 1] - ub;
@@ -160,10 +162,37 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                     const double e = pp.E[el[j] + (int64_t)h * capl[j] + cc];
                     El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -e : e;
                 }
+            // PFU: the uniform primitive pair's data (p, 1/p, P, c and the E of
+            // the item's uniform components) are loaded into vector registers
+            // one iteration ahead instead of through the scalar cache, whose
+            // latency each iteration otherwise waits for.  The opaque zero
+            // makes their addresses lane-dependent for the compiler.
+            constexpr int NUE = PFU ? kDirectMaxU * HU : 1;
+            double nU[6], nEu[NUE];
+            auto load_u = [&](int a) {
+                const double *pa = pu + (int64_t)a * 8 + vzero;
+#pragma unroll
+                for (int t = 0; t < 6; t++) nU[t] = pa[t];
+                const double *ea = eu + (int64_t)a * erow + vzero;
+#pragma unroll
+                for (int t = 0; t < NUE; t++) nEu[t] = t < it.nu * HU ? ea[t] : 0.0;
+            };
+            if constexpr (PFU) load_u(it.a0);
             for (int a = it.a0; a < it.a1; a++) {
-                const double *pa = pu + (int64_t)a * 8;
-                const double p = pa[0], ip = pa[1];
-                const double X = pa[2] - Qx, Y = pa[3] - Qy, Z = pa[4] - Qz;
+                double U[6], Eu[NUE];
+                if constexpr (PFU) {
+#pragma unroll
+                    for (int t = 0; t < 6; t++) U[t] = nU[t];
+#pragma unroll
+                    for (int t = 0; t < NUE; t++) Eu[t] = nEu[t];
+                    if (a + 1 < it.a1) load_u(a + 1);
+                } else {
+                    const double *pa = pu + (int64_t)a * 8;
+#pragma unroll
+                    for (int t = 0; t < 6; t++) U[t] = pa[t];
+                }
+                const double p = U[0], ip = U[1];
+                const double X = U[2] - Qx, Y = U[3] - Qy, Z = U[4] - Qz;
                 const double s = p + q;
                 const double is = 1.0 / s;
                 const double alpha = (p * q) * is;
@@ -172,7 +201,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 boys<L>(T, F);
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
-                const double f = (pa[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
+                const double f = (U[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
                 // hl outer, hu inner: CLB * HU independent FMA chains
                 double Xv[CLB][HU];
 #pragma unroll
@@ -195,7 +224,9 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
 #pragma unroll
                     for (int hu = 0; hu < HU; hu++) Xv[j][hu] *= f;
                 const double *ea = eu + (int64_t)a * erow;
-                for (int u = 0; u < it.nu; u++) {
+#pragma unroll
+                for (int u = 0; u < kDirectMaxU; u++) {
+                    if (u >= it.nu) break;
                     // independent partial sums keep the FMA chains short
                     constexpr int NP = CLB == 1 ? 8 : CLB == 2 ? 4 : 2;
                     double y[CLB][NP];
@@ -205,7 +236,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                         for (int t = 0; t < NP; t++) y[j][t] = 0.0;
 #pragma unroll
                     for (int hu = 0; hu < HU; hu++) {
-                        const double e = ea[u * HU + hu];
+                        const double e = PFU ? Eu[(u * HU + hu) % NUE] : ea[u * HU + hu];
 #pragma unroll
                         for (int j = 0; j < CLB; j++)
                             y[j][hu % NP] = fma(e, Xv[j][hu], y[j][hu % NP]);
@@ -242,7 +273,8 @@ constexpr int pow2ceil(int x)
 // folded into the lane E), sums it over the 64 lanes through LDS, and lane u
 // then contracts the sum with the E of its own uniform component u.  One LDS
 // reduction per (a, pass).
-template <int LU, int LL, int CLB>
+// PF: lane data of the next chunk loaded one chunk ahead (kShellPrefetch)
+template <int LU, int LL, int CLB, bool PF>
 __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                                                       const ShellItem *__restrict__ items,
                                                       const double *__restrict__ pu_all,
@@ -278,6 +310,25 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
         double acc[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) acc[j] = 0.0;
+        // lane data of the chunk in flight (PF)
+        double nq = 0.0, niq = 0.0, nQx = 0.0, nQy = 0.0, nQz = 0.0, nck = 0.0, nE[CLB][HL];
+        auto load_lane = [&](int c0) {
+            const int c = c0 + lane;
+            const bool live = c < nl;
+            const int cc = live ? c : c0;
+            nq = pp.p[ol + cc];
+            niq = pp.ip[ol + cc];
+            nQx = pp.Px[ol + cc];
+            nQy = pp.Py[ol + cc];
+            nQz = pp.Pz[ol + cc];
+            nck = live ? pp.c[ol + cc] : 0.0;
+#pragma unroll
+            for (int j = 0; j < CLB; j++)
+#pragma unroll
+                for (int h = 0; h < HL; h++)
+                    nE[j][h] = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+        };
+        if constexpr (PF) load_lane(0);
         for (int a = it.a0; a < it.a1; a++) {
             const double *pa = pu + (int64_t)a * 8;
             const double p = pa[0], ip = pa[1], Px = pa[2], Py = pa[3], Pz = pa[4], cb = pa[5];
@@ -291,10 +342,34 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 const int c = c0 + lane;
                 const bool live = c < nl;
                 const int cc = live ? c : c0; // in-range index for idle lanes
-                const double q = pp.p[ol + cc], iq = pp.ip[ol + cc];
-                const double X = Px - pp.Px[ol + cc], Y = Py - pp.Py[ol + cc],
-                             Z = Pz - pp.Pz[ol + cc];
-                const double ck = live ? pp.c[ol + cc] : 0.0;
+                double q, iq, Qx, Qy, Qz, ck, Er[CLB][HL];
+                if constexpr (PF) {
+                    // this chunk's lane data were loaded one chunk ahead; the
+                    // next chunk's (or, after the last, the first chunk of the
+                    // next uniform primitive pair) go in flight now, behind
+                    // this chunk's Boys, R and X work
+                    q = nq;
+                    iq = niq;
+                    Qx = nQx;
+                    Qy = nQy;
+                    Qz = nQz;
+                    ck = nck;
+#pragma unroll
+                    for (int j = 0; j < CLB; j++)
+#pragma unroll
+                        for (int h = 0; h < HL; h++) Er[j][h] = nE[j][h];
+                    // (one chunk in all: its data stay in the prefetch set)
+                    const int n0 = c0 + 64 < nl ? c0 + 64 : 0;
+                    if (c0 + 64 < nl || (a + 1 < it.a1 && nl > 64)) load_lane(n0);
+                } else {
+                    q = pp.p[ol + cc];
+                    iq = pp.ip[ol + cc];
+                    Qx = pp.Px[ol + cc];
+                    Qy = pp.Py[ol + cc];
+                    Qz = pp.Pz[ol + cc];
+                    ck = live ? pp.c[ol + cc] : 0.0;
+                }
+                const double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
                 const double s = p + q;
                 const double is = 1.0 / s;
                 const double alpha = (p * q) * is;
@@ -312,8 +387,11 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                         // (cl + j < it.cl1 always holds; as a condition it keeps
                         // the compiler from issuing these loads one at a time
                         // next to their uses, which serialized their latency)
-                        const double e =
-                            cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                        double e;
+                        if constexpr (PF)
+                            e = Er[j][h];
+                        else
+                            e = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
                         El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -(e * f) : e * f;
                     }
 #pragma unroll
@@ -423,16 +501,38 @@ constexpr int kShellClbRed[5][5] = {{1, 3, 3, 4, 2},
                                     {1, 3, 3, 1, 1},
                                     {1, 2, 1, 1, 1}};
 
+// Reduction-mode classes whose lane data are loaded one chunk ahead: those
+// at one wave per SIMD anyway, with room for a second set of lane registers
+// without spilling (single-component passes of order-4 quartets stay at
+// two waves per SIMD without it and keep that).
+constexpr bool kShellPrefetch[5][5] = {{false, false, false, false, false},
+                                       {false, false, false, true, true},
+                                       {false, false, true, true, false},
+                                       {false, true, false, true, false},
+                                       {false, true, true, false, false}};
+constexpr bool shell_prefetch(int LU, int LL, int B)
+{
+    return kShellPrefetch[LU][LL] && !(B == 1 && LU + LL <= 4);
+}
+// Direct-mode classes whose uniform data are loaded one iteration ahead into
+// vector registers (uniform side of order 0 or 1: at most 18 doubles): the
+// ones already at one wave per SIMD without it (lane order 3 and 4) and
+// still free of spills with it.
+constexpr bool shell_prefetch_u(int LU, int LL, int B)
+{
+    return LU <= 1 && LL >= 3 && !(LL == 4 && LU + B >= 4);
+}
+
 template <int LU, int LL, int B>
 void launch_shell_b(bool red, unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
                     const ShellItem *items, double *part)
 {
     if (red)
-        hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B>), dim3(grid), dim3(64), 0, st, pp, sd,
-                           items, sd.pu, sd.eu2, part);
+        hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B, shell_prefetch(LU, LL, B)>), dim3(grid),
+                           dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu2, part);
     else
-        hipLaunchKernelGGL((k_eri_shell<LU, LL, B>), dim3(grid), dim3(64), 0, st, pp, sd, items,
-                           sd.pu, sd.eu, part);
+        hipLaunchKernelGGL((k_eri_shell<LU, LL, B, shell_prefetch_u(LU, LL, B)>), dim3(grid),
+                           dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu, part);
 }
 
 // instances up to the class's batch limit only (the larger ones are never
