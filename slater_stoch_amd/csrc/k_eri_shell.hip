@@ -673,12 +673,13 @@ double item_cost(const zv_context *c, int su, int sl)
     return cost;
 }
 
-// Host threads for the planner: up to 8, one for short lists.
+// Host threads for the planner: up to 16 (the host share of one GPU), one
+// for short lists.
 int plan_threads(size_t n)
 {
     if (n < 20000) return 1;
     const unsigned hw = std::thread::hardware_concurrency();
-    return (int)std::max(1u, std::min(8u, hw ? hw : 1u));
+    return (int)std::max(1u, std::min(16u, hw ? hw : 1u));
 }
 
 // run f(t) for t = 0 .. nt-1, on nt - 1 extra threads and this one; a
