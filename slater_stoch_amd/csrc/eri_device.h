@@ -47,6 +47,7 @@ struct PP {
     const int64_t *off, *eoff;          // per pair
     const int *cap;                     // n_gauss(i) n_gauss(k)
     const int *L;                       // Hermite order per pair
+    const double *btab;                 // Boys Taylor table [L][k][i] (boys_table)
 };
 
 // ------------------------------------------------------------------ Boys
@@ -94,6 +95,58 @@ template <int L> __device__ inline void boys(double T, double *F)
             for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
         }
     }
+}
+
+// Tabulated Boys function (the shell kernels): for T < 36 a Taylor
+// expansion of F_L about the nearest grid point T_i = i / 2,
+//   F_L(T_i + d) = sum_k (-1)^k F_{L+k}(T_i) d^k / k!,  |d| <= 1/4, k <= 14
+// (remainder below 1e-21 of F_L), then the downward recursion
+//   F_m = (2 T F_{m+1} + exp(-T)) / (2m + 1),
+// which is stable; the table holds (-1)^k F_{L+k}(T_i) / k! for L <= 8,
+// computed on the host in extended precision from the series
+// F_m(T) = exp(-T) sum_j (2T)^j / ((2m+1)(2m+3)...(2m+2j+1)) (zv_boys_table).
+// From T = 36 on the branch of boys<L> (F_0 = sqrt(pi/T)/2, erf rounding to
+// 1, upward recursion).  Relative error below 1e-15 against the series in
+// extended precision; one code path for all T < 36, where boys<L> splits at
+// T = 1 and evaluates erf above it.  lt: the kernel's LDS copy of the
+// L-slice, [k][i].
+constexpr int kBoysK = 14, kBoysN = 73; // orders 0..kBoysK, grid 0, 0.5, .., 36
+constexpr int kBoysLmax = 8;
+constexpr int kBoysSlice = (kBoysK + 1) * kBoysN;
+
+template <int L> __device__ inline void boys_table(double T, const double *lt, double *F)
+{
+    if (T < 36.0) {
+        const int i = (int)(T * 2.0 + 0.5);
+        const double d = T - 0.5 * (double)i;
+        double s = lt[kBoysK * kBoysN + i];
+#pragma unroll
+        for (int k = kBoysK - 1; k >= 0; k--) s = fma(s, d, lt[k * kBoysN + i]);
+        F[L] = s;
+        if (L > 0) {
+            const double e = exp(-T);
+#pragma unroll
+            for (int m = L - 1; m >= 0; m--)
+                F[m] = fma(2.0 * T, F[m + 1], e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        double u_inv = rsqrt(T);
+        F[0] = (0.5 * u_inv) * kSqrtPi;
+        if (L > 0) {
+            double e = T > 80.0 ? 0.0 : exp(-T);
+            double tri = (0.5 * u_inv) * u_inv;
+#pragma unroll
+            for (int m = 0; m < L; m++) F[m + 1] = (F[m] * (2 * m + 1) - e) * tri;
+        }
+    }
+}
+
+// the L-slice of the table into LDS (one workgroup of 64 threads)
+template <int L> __device__ inline void boys_table_load(const double *btab, double *lt)
+{
+    const double *src = btab + (size_t)L * kBoysSlice;
+    for (int t = threadIdx.x; t < kBoysSlice; t += 64) lt[t] = src[t];
+    __syncthreads();
 }
 
 // generic-L version (runtime loop)

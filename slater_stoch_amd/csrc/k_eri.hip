@@ -367,9 +367,58 @@ bool launch_class_t(int LB, int LK, unsigned grid, hipStream_t st, PP pp, QSrc S
 
 // ====================================================================== host
 // Device pointers of the primitive-pair tables inside b_pp, b_ppE, b_ppcnt.
+// The Boys Taylor table of boys_table (eri_device.h), computed once per
+// process in extended precision (x87 long double on the host):
+// [L][k][i] = (-1)^k F_{L+k}(i / 2) / k!, L <= kBoysLmax, k <= kBoysK,
+// F_m(T) = exp(-T) sum_j (2T)^j / ((2m+1)(2m+3)...(2m+2j+1)), all terms
+// positive, summed until they fall below 1e-22 of the sum.
+static const std::vector<double> &boys_table_host()
+{
+    static std::vector<double> tab;
+    if (!tab.empty()) return tab;
+    constexpr int NM = kBoysLmax + kBoysK + 1;
+    std::vector<long double> F((size_t)NM * kBoysN);
+    for (int m = 0; m < NM; m++)
+        for (int i = 0; i < kBoysN; i++) {
+            const long double T = 0.5L * i;
+            long double term = 1.0L / (2 * m + 1), sum = term;
+            for (int j = 1; j < 1000; j++) {
+                term *= 2.0L * T / (2 * m + 2 * j + 1);
+                sum += term;
+                if (term < 1e-22L * sum) break;
+            }
+            F[(size_t)m * kBoysN + i] = sum * expl(-T);
+        }
+    std::vector<double> t((size_t)(kBoysLmax + 1) * kBoysSlice);
+    for (int L = 0; L <= kBoysLmax; L++) {
+        long double fact = 1.0L;
+        for (int k = 0; k <= kBoysK; k++) {
+            if (k > 0) fact *= k;
+            for (int i = 0; i < kBoysN; i++)
+                t[(size_t)L * kBoysSlice + (size_t)k * kBoysN + i] =
+                    (double)(((k & 1) ? -1.0L : 1.0L) * F[(size_t)(L + k) * kBoysN + i] / fact);
+        }
+    }
+    tab.swap(t);
+    return tab;
+}
+
+// the table on the context's device (once per context)
+static int ensure_boys_table(zv_context *c)
+{
+    const std::vector<double> &t = boys_table_host();
+    ZV_CHECK(zv_ensure(c, c->b_btab, sizeof(double) * t.size()));
+    if (c->btab_ready) return ZV_OK;
+    ZV_HIPCHECK(c, hipMemcpyAsync(c->b_btab.p, t.data(), sizeof(double) * t.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+    c->btab_ready = true;
+    return ZV_OK;
+}
+
 static PP pp_view(zv_context *c)
 {
     PP pp;
+    pp.btab = (const double *)c->b_btab.p;
     const int np = c->npair;
     double *base = (double *)c->b_pp.p;
     size_t T = (size_t)c->pp_total;
@@ -395,6 +444,8 @@ static PP pp_view(zv_context *c)
 static int ensure_primpairs(zv_context *c, PP &pp)
 {
     int np = c->npair;
+    ZV_CHECK(ensure_boys_table(c));
+    pp.btab = (const double *)c->b_btab.p;
     // layout inside b_pp: 6 planes of pp_total doubles; b_ppE: e_total doubles;
     // b_ppcnt: cnt[np] ints, off[np], eoff[np] int64, cap[np], L[np] ints
     ZV_CHECK(zv_ensure(c, c->b_pp, sizeof(double) * 6 * (size_t)c->pp_total));

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
 // Direct mode (few uniform components): every lane contracts its X with the
 // E of every uniform component, scalar E operands, nu * HU FMAs per lane and
 // primitive quartet; per-lane sums over the uniform components live in LDS.
-template <int LU, int LL, int CLB, bool PFU>
+template <int LU, int LL, int CLB, bool PFU, bool TB>
 __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                                                   const ShellItem *__restrict__ items,
                                                   const double *__restrict__ pu_all,
@@ -123,6 +123,8 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
 {
     constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
     __shared__ double accs[kDirectMaxU * CLB * 64]; // it.nu <= kDirectMaxU (host-checked)
+    __shared__ double lbt[TB ? kBoysSlice : 1];     // Boys Taylor table, this L
+    if constexpr (TB) boys_table_load<L>(pp.btab, lbt);
     const int lane = threadIdx.x;
     int vzero = 0;
     if constexpr (PFU) asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
@@ -198,7 +200,10 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 const double alpha = (p * q) * is;
                 const double T = alpha * ((X * X + Y * Y) + Z * Z);
                 double F[L + 1];
-                boys<L>(T, F);
+                if constexpr (TB)
+                    boys_table<L>(T, lbt, F);
+                else
+                    boys<L>(T, F);
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
                 const double f = (U[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
@@ -274,7 +279,7 @@ constexpr int pow2ceil(int x)
 // then contracts the sum with the E of its own uniform component u.  One LDS
 // reduction per (a, pass).
 // PF: lane data of the next chunk loaded one chunk ahead (kShellPrefetch)
-template <int LU, int LL, int CLB, bool PF>
+template <int LU, int LL, int CLB, bool PF, bool TB>
 __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                                                       const ShellItem *__restrict__ items,
                                                       const double *__restrict__ pu_all,
@@ -285,6 +290,8 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
     constexpr int NR = CLB * HU; // rows summed over the wave
     constexpr int S = NR | 1;    // odd lane stride of the transpose buffer
     __shared__ double lds[64 * S + NR];
+    __shared__ double lbt[TB ? kBoysSlice : 1]; // Boys Taylor table, this L
+    if constexpr (TB) boys_table_load<L>(pp.btab, lbt);
     const int lane = threadIdx.x;
     const ShellItem it = items[blockIdx.x];
     const int ub = sd.coff[it.su], lb = sd.coff[it.sl];
@@ -375,7 +382,10 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 const double alpha = (p * q) * is;
                 const double T = alpha * ((X * X + Y * Y) + Z * Z);
                 double F[L + 1];
-                boys<L>(T, F);
+                if constexpr (TB)
+                    boys_table<L>(T, lbt, F);
+                else
+                    boys<L>(T, F);
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
                 const double f = (cb * ck) * (((kPi52 * ip) * iq) * sqrt(is));
@@ -514,6 +524,22 @@ constexpr bool shell_prefetch(int LU, int LL, int B)
 {
     return kShellPrefetch[LU][LL] && !(B == 1 && LU + LL <= 4);
 }
+// Instances that evaluate the Boys function from the Taylor table in LDS
+// (boys_table, 8.8 KB per workgroup) instead of erf and the series (boys):
+// all whose LDS then still allows two workgroups of one wave per SIMD (20 KB
+// each), and the reduction kernels of order 4 and up, which run one wave
+// per SIMD anyway, up to 40 KB (four per CU).
+constexpr int shell_lds_bytes(int LU, int B, bool red)
+{
+    const int HU = hcount(LU), NR = B * HU, S = NR | 1;
+    return red ? (64 * S + NR) * 8 : kDirectMaxU * B * 64 * 8;
+}
+constexpr bool shell_boys_table(int LU, int LL, int B, bool red)
+{
+    const int bytes = shell_lds_bytes(LU, B, red) + kBoysSlice * 8;
+    return LU + LL <= kBoysLmax &&
+           (bytes <= 20480 || (red && LU + LL >= 4 && bytes <= 40960));
+}
 // Direct-mode classes whose uniform data are loaded one iteration ahead into
 // vector registers (uniform side of order 0 or 1: at most 18 doubles): the
 // ones already at one wave per SIMD without it (lane order 3 and 4) and
@@ -528,11 +554,13 @@ void launch_shell_b(bool red, unsigned grid, hipStream_t st, const PP &pp, const
                     const ShellItem *items, double *part)
 {
     if (red)
-        hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B, shell_prefetch(LU, LL, B)>), dim3(grid),
-                           dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu2, part);
+        hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B, shell_prefetch(LU, LL, B),
+                                            shell_boys_table(LU, LL, B, true)>),
+                           dim3(grid), dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu2, part);
     else
-        hipLaunchKernelGGL((k_eri_shell<LU, LL, B, shell_prefetch_u(LU, LL, B)>), dim3(grid),
-                           dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu, part);
+        hipLaunchKernelGGL((k_eri_shell<LU, LL, B, shell_prefetch_u(LU, LL, B),
+                                        shell_boys_table(LU, LL, B, false)>),
+                           dim3(grid), dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu, part);
 }
 
 // instances up to the class's batch limit only (the larger ones are never

@@ -84,6 +84,8 @@ struct zv_context {
     bool res_valid[3] = {false, false, false};
     DevBuf b_mpart;                    // estimator partial sums of sample splits 1..n-1
     DevBuf b_pp;                       // primitive-pair data
+    DevBuf b_btab;                     // Boys Taylor table (zv_boys_table)
+    bool btab_ready = false;
     DevBuf b_ppE;                      // primitive-pair Hermite coefficients
     DevBuf b_ppcnt;                    // kept primitive pairs per AO pair
     DevBuf b_tri;                      // triangle-order ERIs
