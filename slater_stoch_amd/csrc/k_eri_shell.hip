@@ -498,7 +498,9 @@ __global__ void k_zero(double *x, int64_t n)
 // sums stay in the register file without spilling, and by shell-pair sizes:
 // an order-0 pair has 1 component, order 1 has 3 (p x s), order 2 has 9
 // (p x p) or 6 (d x s), so 3 covers them in as few passes as 4 does with
-// fewer registers.
+// fewer registers.  (2, 2) in reduction mode runs passes of 2: with the
+// tabulated Boys function they fit 236 registers, two waves per SIMD, which
+// beats passes of 3 at one wave (949 against 1088 ms on the 200-AO molecule).
 constexpr int kClbMax = 4;
 constexpr int kShellClb[5][5] = {{1, 3, 3, 4, 4},
                                  {1, 3, 3, 4, 3},
@@ -507,7 +509,7 @@ constexpr int kShellClb[5][5] = {{1, 3, 3, 4, 4},
                                  {1, 3, 2, 1, 1}};
 constexpr int kShellClbRed[5][5] = {{1, 3, 3, 4, 2},
                                     {1, 3, 3, 3, 1},
-                                    {1, 3, 3, 2, 1},
+                                    {1, 3, 2, 2, 1},
                                     {1, 3, 3, 1, 1},
                                     {1, 2, 1, 1, 1}};
 
@@ -517,7 +519,7 @@ constexpr int kShellClbRed[5][5] = {{1, 3, 3, 4, 2},
 // two waves per SIMD without it and keep that).
 constexpr bool kShellPrefetch[5][5] = {{false, false, false, false, false},
                                        {false, false, false, true, true},
-                                       {false, false, true, true, false},
+                                       {false, false, false, true, false},
                                        {false, true, false, true, false},
                                        {false, true, true, false, false}};
 constexpr bool shell_prefetch(int LU, int LL, int B)
@@ -755,6 +757,7 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
         const int k = 5 * LU + LL;
         const double HU = hc(LU);
         const int step = uniform_step(ncu);
+        bool br_counted = false; // Boys and R counted once per shell quartet
         for (int cu0 = 0; cu0 < ncu; cu0 += step) {
             const int nu = std::min(step, ncu - cu0);
             const bool red = uniform_red(ncu, nu);
@@ -779,14 +782,15 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
                     const int ns =
                         (int)std::min<double>(npu, std::max(1.0, std::ceil(cr / budget)));
                     pp.groups.push_back(ShellGroup{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pp.npart});
-                    // algorithmic flops: Boys and R once per primitive quartet;
-                    // per lane component the X contraction, then either nu
-                    // direct dot products per primitive quartet or the sum over
-                    // lane primitives per primitive quartet plus nu dot
-                    // products per uniform primitive
+                    // algorithmic flops: Boys and R once per primitive quartet
+                    // (however many lane passes redo them); per lane component
+                    // the X contraction, then either nu direct dot products per
+                    // primitive quartet or the sum over lane primitives per
+                    // primitive quartet plus nu dot products per uniform primitive
                     const double pq = (double)npu * npl;
                     pp.units[b] += pq * nlr;
-                    double fl = pq * (boys_r_flops(L) + nlr * x_flops(LU, LL));
+                    double fl = pq * ((br_counted ? 0.0 : boys_r_flops(L)) + nlr * x_flops(LU, LL));
+                    br_counted = true;
                     fl += red ? nlr * (pq * HU + (double)npu * 2.0 * nu * HU)
                               : nlr * pq * 2.0 * nu * HU;
                     pp.flops[b] += fl;
