@@ -162,8 +162,9 @@ def roofline_eri(res):
     Schwarz phase run side by side on the library's five streams and are
     timed together (HIP events around the fork and join on the library's
     stream).  Algorithmic FP64 flops (Boys values and R table once per
-    primitive quartet, Hermite contractions per component, as counted by the
-    library) over the phase time; a "launch" is one phase's set of buckets."""
+    primitive quartet, Hermite contractions per component over its nonzero
+    Hermite coefficients, as counted by the library) over the phase time; a
+    "launch" is one phase's set of buckets."""
     st = res["eri_stats"]
     flops = sum(v["flops"] for v in st.values())
     ms, nset = res["an_timings"]["analytic_eri"]

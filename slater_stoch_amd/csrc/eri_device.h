@@ -39,6 +39,71 @@ struct HDecode {
 };
 constexpr HDecode kH{};
 
+// ------------------------------------------------------------------ E sparsity
+// Structural zeros of a pair's Hermite coefficients.  For AOs with Cartesian
+// powers a and b, E_tuv = E^x_t E^y_u E^z_v is nonzero only for t <= a_x +
+// b_x, u <= a_y + b_y, v <= a_z + b_z (a box over the per-axis sums S = a +
+// b); an r^2-type AO (i_type 2) sums its three variants with 2 added to one
+// power, so a pair with nr such AOs has the union of the boxes of S plus 2
+// on nr axes.  HermPatterns lists, for every pair order L <= ZV_LMAX_T, each
+// nonzero set that can occur (nr = 0, 1, 2 and |S| + 2 nr = L), with its
+// Hermite indices in ascending order.  k_primpairs writes exact zeros
+// outside the set, so sums over it equal sums over all indices.
+constexpr int kPatMax = 22, kNzMax = 25;
+constexpr uint64_t herm_box(int bx, int by, int bz)
+{
+    uint64_t m = 0;
+    for (int t = 0; t <= bx; t++)
+        for (int u = 0; u <= by; u++)
+            for (int v = 0; v <= bz; v++) m |= 1ull << hidx(t, u, v);
+    return m;
+}
+// nonzero set of per-axis sums S with nr r^2-type AOs
+constexpr uint64_t herm_union(int sx, int sy, int sz, int nr)
+{
+    if (nr == 0) return herm_box(sx, sy, sz);
+    uint64_t m = 0;
+    for (int v = 0; v < 3; v++)
+        m |= herm_union(sx + (v == 0 ? 2 : 0), sy + (v == 1 ? 2 : 0), sz + (v == 2 ? 2 : 0),
+                        nr - 1);
+    return m;
+}
+struct HermPatterns {
+    int n[ZV_LMAX_T + 1] = {};
+    uint64_t mask[ZV_LMAX_T + 1][kPatMax] = {};
+    int nz[ZV_LMAX_T + 1][kPatMax] = {};
+    int h[ZV_LMAX_T + 1][kPatMax][kNzMax] = {};
+    int nzmax[ZV_LMAX_T + 1] = {};
+    constexpr HermPatterns()
+    {
+        for (int L = 0; L <= ZV_LMAX_T; L++)
+            for (int nr = 0; 2 * nr <= L; nr++) {
+                const int s = L - 2 * nr;
+                for (int sx = s; sx >= 0; sx--)
+                    for (int sy = s - sx; sy >= 0; sy--) {
+                        const uint64_t m = herm_union(sx, sy, s - sx - sy, nr);
+                        bool seen = false;
+                        for (int p = 0; p < n[L]; p++) seen = seen || mask[L][p] == m;
+                        if (seen) continue;
+                        const int p = n[L]++;
+                        mask[L][p] = m;
+                        for (int hh = 0; hh < 64; hh++)
+                            if ((m >> hh) & 1) h[L][p][nz[L][p]++] = hh;
+                        if (nz[L][p] > nzmax[L]) nzmax[L] = nz[L][p];
+                    }
+            }
+    }
+    // pattern index of a nonzero set of order L, -1 if it is not listed
+    constexpr int find(int L, uint64_t m) const
+    {
+        for (int p = 0; p < n[L]; p++)
+            if (mask[L][p] == m) return p;
+        return -1;
+    }
+};
+constexpr HermPatterns kHP{};
+static_assert(kHP.n[ZV_LMAX_T] <= kPatMax && kHP.nzmax[ZV_LMAX_T] <= kNzMax, "pattern table");
+
 // ------------------------------------------------------------------ pair data
 struct PP {
     double *p, *ip, *Px, *Py, *Pz, *c; // [pp_total], compacted per AO pair

@@ -23,11 +23,18 @@
 // the 64 lanes.  Per-lane sums over the uniform components live in LDS;
 // each item writes its partial sums, and one scatter kernel adds the slices,
 // applies the Schwarz test per component quartet and writes triangle order.
+//
+// E^l of one lane component is nonzero only on a pattern of Hermite indices
+// fixed by the AOs' Cartesian powers (HermPatterns); the classes listed in
+// kShellSparse read the pattern's rows of a compacted copy of E and run the
+// X contraction over them only, the same terms in the same order minus
+// exact zeros.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "eri_device.h"
@@ -45,6 +52,8 @@ struct SPD {
     const double *pu;     // [a][8]: p, 1/p, Px, Py, Pz, c, 0, 0
     const double *eu;     // [a][component][h]  (scalar reads, direct mode)
     const double *eu2;    // [a][h][component]  (lane-per-component reads, reduction mode)
+    const int *cpat;      // per component: Hermite pattern index | nonzero count << 8
+    const double *ec;     // lane-side E: the pattern's nonzero rows, offsets and stride of E
 };
 
 struct ShellItem {
@@ -106,6 +115,65 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
     }
 }
 
+// Lane-side E without its structural zeros: per component (AO pair), the
+// rows of the nonzero Hermite indices of its pattern, in ascending order, at
+// the pair's E offset with the same row stride (one block per component).
+__global__ __launch_bounds__(256) void k_compact_e(PP pp, SPD sd, double *ec)
+{
+    const int q = blockIdx.x;
+    const int P = sd.comp[q];
+    const int L = pp.L[P];
+    if (L > ZV_LMAX_T) return;
+    const int pid = sd.cpat[q] & 0xff, nz = sd.cpat[q] >> 8;
+    const int n = pp.cnt[P], cap = pp.cap[P];
+    const int64_t eo = pp.eoff[P];
+    for (int t = threadIdx.x; t < nz * n; t += blockDim.x) {
+        const int k = t / n, a = t - k * n;
+        ec[eo + (int64_t)k * cap + a] = pp.E[eo + (int64_t)kHP.h[L][pid][k] * cap + a];
+    }
+}
+
+// Calls f(std::integral_constant<int, p>) for the runtime pattern index p of
+// order LL (a chain of uniform compares), so the body sees the pattern's
+// nonzero Hermite indices as constants.
+template <int LL, int I = 0, class F> __device__ __forceinline__ void pat_dispatch(int p, F &&f)
+{
+    if constexpr (I < kHP.n[LL]) {
+        if (p == I)
+            f(std::integral_constant<int, I>{});
+        else
+            pat_dispatch<LL, I + 1>(p, f);
+    }
+}
+
+// X[hu] over one lane component: the sum over the nonzero Hermite indices
+// hl of its pattern pat, in ascending order, of (-1)^(t+u+v of hl) sc(E[k])
+// R_{hu+hl}, E[k] the k-th nonzero coefficient; with INIT the first term
+// sets X (hl = 0, even), else every term is added.  The skipped terms are
+// exact zeros, so X equals the sum over all indices.
+template <int LU, int LL, bool INIT, int NZ, class Sc>
+__device__ __forceinline__ void lane_contract(int pat, const double (&E)[NZ], const double *R,
+                                              double (&X)[hcount(LU)], Sc sc)
+{
+    constexpr int HU = hcount(LU);
+    auto term = [&](int k, int hl) {
+        const double ek = sc(E[k]);
+        const double e = ((kH.t[hl] + kH.u[hl] + kH.v[hl]) & 1) ? -ek : ek;
+#pragma unroll
+        for (int hu = 0; hu < HU; hu++) {
+            const double r =
+                R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl], kH.v[hu] + kH.v[hl])];
+            X[hu] = INIT && k == 0 ? e * r : fma(e, r, X[hu]);
+        }
+    };
+    pat_dispatch<LL>(pat, [&](auto PI) {
+        constexpr int P = decltype(PI)::value;
+        static_assert(kHP.h[LL][P][0] == 0, "pattern order");
+#pragma unroll
+        for (int k = 0; k < kHP.nz[LL][P]; k++) term(k, kHP.h[LL][P][k]);
+    });
+}
+
 // The uniform-side arrays and the output are separate restrict arguments so
 // the compiler may read the former through the scalar cache (no store of
 // the kernel can reach them).  CLB lane-side components are processed per
@@ -114,14 +182,15 @@ __global__ __launch_bounds__(256) void k_shellpairs(PP pp, SPD sd, double *pu, d
 // Direct mode (few uniform components): every lane contracts its X with the
 // E of every uniform component, scalar E operands, nu * HU FMAs per lane and
 // primitive quartet; per-lane sums over the uniform components live in LDS.
-template <int LU, int LL, int CLB, bool PFU, bool TB>
+template <int LU, int LL, int CLB, bool PFU, bool TB, bool SP>
 __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                                                   const ShellItem *__restrict__ items,
                                                   const double *__restrict__ pu_all,
                                                   const double *__restrict__ eu_all,
                                                   double *__restrict__ part)
 {
-    constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
+    constexpr int HU = hcount(LU), L = LU + LL, HR = hcount(L);
+    constexpr int NZ = SP ? kHP.nzmax[LL] : hcount(LL); // lane E per component
     __shared__ double accs[kDirectMaxU * CLB * 64]; // it.nu <= kDirectMaxU (host-checked)
     __shared__ double lbt[TB ? kBoysSlice : 1];     // Boys Taylor table, this L
     if constexpr (TB) boys_table_load<L>(pp.btab, lbt);
@@ -141,12 +210,15 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
     // the item's lane components are whole passes of CLB (host-checked)
     for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
         int64_t el[CLB];
-        int capl[CLB];
+        int capl[CLB], pat[CLB], nzl[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) {
             const int Pl = sd.comp[lb + cl + j];
             el[j] = pp.eoff[Pl];
             capl[j] = pp.cap[Pl];
+            const int cp = sd.cpat[lb + cl + j];
+            pat[j] = cp & 0xff;
+            nzl[j] = cp >> 8;
         }
         for (int u = 0; u < it.nu * CLB; u++) accs[u * 64 + lane] = 0.0;
         for (int c0 = 0; c0 < nl; c0 += 64) {
@@ -156,13 +228,18 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
             const double q = pp.p[ol + cc], iq = pp.ip[ol + cc];
             const double Qx = pp.Px[ol + cc], Qy = pp.Py[ol + cc], Qz = pp.Pz[ol + cc];
             const double ck = live ? pp.c[ol + cc] : 0.0;
-            double El[CLB][HL];
+            // the lane E of each component (SP: its nonzero ones, pattern order)
+            double El[CLB][NZ];
 #pragma unroll
             for (int j = 0; j < CLB; j++)
 #pragma unroll
-                for (int h = 0; h < HL; h++) {
-                    const double e = pp.E[el[j] + (int64_t)h * capl[j] + cc];
-                    El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -e : e;
+                for (int k = 0; k < NZ; k++) {
+                    if constexpr (SP) {
+                        El[j][k] = k < nzl[j] ? sd.ec[el[j] + (int64_t)k * capl[j] + cc] : 0.0;
+                    } else {
+                        const double e = pp.E[el[j] + (int64_t)k * capl[j] + cc];
+                        El[j][k] = ((kH.t[k] + kH.u[k] + kH.v[k]) & 1) ? -e : e;
+                    }
                 }
             // PFU: the uniform primitive pair's data (p, 1/p, P, c and the E of
             // the item's uniform components) are loaded into vector registers
@@ -207,23 +284,30 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
                 const double f = (U[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
-                // hl outer, hu inner: CLB * HU independent FMA chains
                 double Xv[CLB][HU];
+                if constexpr (SP) {
+                    // HU independent FMA chains per lane component
 #pragma unroll
-                for (int j = 0; j < CLB; j++)
+                    for (int j = 0; j < CLB; j++)
+                        lane_contract<LU, LL, true>(pat[j], El[j], R, Xv[j],
+                                                    [](double e) { return e; });
+                } else {
+                    // hl outer, hu inner: CLB * HU independent FMA chains
 #pragma unroll
-                    for (int hu = 0; hu < HU; hu++)
-                        Xv[j][hu] = El[j][0] * R[hu];
+                    for (int j = 0; j < CLB; j++)
 #pragma unroll
-                for (int hl = 1; hl < HL; hl++)
+                        for (int hu = 0; hu < HU; hu++) Xv[j][hu] = El[j][0] * R[hu];
 #pragma unroll
-                    for (int hu = 0; hu < HU; hu++)
+                    for (int hl = 1; hl < NZ; hl++)
 #pragma unroll
-                        for (int j = 0; j < CLB; j++)
-                            Xv[j][hu] = fma(El[j][hl],
-                                            R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
-                                                   kH.v[hu] + kH.v[hl])],
-                                            Xv[j][hu]);
+                        for (int hu = 0; hu < HU; hu++)
+#pragma unroll
+                            for (int j = 0; j < CLB; j++)
+                                Xv[j][hu] = fma(El[j][hl],
+                                                R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
+                                                       kH.v[hu] + kH.v[hl])],
+                                                Xv[j][hu]);
+                }
 #pragma unroll
                 for (int j = 0; j < CLB; j++)
 #pragma unroll
@@ -272,6 +356,18 @@ constexpr int pow2ceil(int x)
     return p;
 }
 
+// Reduction-mode batch size per class compiled for at least two waves per
+// SIMD (the register allocator then fits 256 registers, spilling if it
+// must); 0: none.  (2, 1) with sparse passes of 3 needs 260 registers
+// otherwise; at two waves it spills three (554 against 595 ms for the
+// dense passes at 254 registers, 908 ms at one wave).
+constexpr int kShellTwoWaves[5][5] = {{0, 0, 0, 0, 0},
+                                      {0, 0, 0, 0, 0},
+                                      {0, 3, 0, 0, 0},
+                                      {0, 0, 0, 0, 0},
+                                      {0, 0, 0, 0, 0}};
+constexpr int shell_waves(int LU, int LL, int B) { return kShellTwoWaves[LU][LL] == B ? 2 : 1; }
+
 // Reduction mode (nu >= 4): the uniform E does not depend on the lane
 // primitive pair, so for each uniform primitive pair a the wave first forms
 // X_hu = sum over all lane primitive pairs (every lane chunk, the prefactor
@@ -279,14 +375,16 @@ constexpr int pow2ceil(int x)
 // then contracts the sum with the E of its own uniform component u.  One LDS
 // reduction per (a, pass).
 // PF: lane data of the next chunk loaded one chunk ahead (kShellPrefetch)
-template <int LU, int LL, int CLB, bool PF, bool TB>
-__global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
+template <int LU, int LL, int CLB, bool PF, bool TB, bool SP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(shell_waves(LU, LL, CLB))))
+void k_eri_shell_red(PP pp, SPD sd,
                                                       const ShellItem *__restrict__ items,
                                                       const double *__restrict__ pu_all,
                                                       const double *__restrict__ eu2_all,
                                                       double *__restrict__ part)
 {
-    constexpr int HU = hcount(LU), HL = hcount(LL), L = LU + LL, HR = hcount(L);
+    constexpr int HU = hcount(LU), L = LU + LL, HR = hcount(L);
+    constexpr int NZ = SP ? kHP.nzmax[LL] : hcount(LL); // lane E per component
     constexpr int NR = CLB * HU; // rows summed over the wave
     constexpr int S = NR | 1;    // odd lane stride of the transpose buffer
     __shared__ double lds[64 * S + NR];
@@ -307,18 +405,21 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
     // the item's lane components are whole passes of CLB (host-checked)
     for (int cl = it.cl0; cl < it.cl1; cl += CLB) {
         int64_t el[CLB];
-        int capl[CLB];
+        int capl[CLB], pat[CLB], nzl[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) {
             const int Pl = sd.comp[lb + cl + j];
             el[j] = pp.eoff[Pl];
             capl[j] = pp.cap[Pl];
+            const int cp = sd.cpat[lb + cl + j];
+            pat[j] = cp & 0xff;
+            nzl[j] = cp >> 8;
         }
         double acc[CLB];
 #pragma unroll
         for (int j = 0; j < CLB; j++) acc[j] = 0.0;
         // lane data of the chunk in flight (PF)
-        double nq = 0.0, niq = 0.0, nQx = 0.0, nQy = 0.0, nQz = 0.0, nck = 0.0, nE[CLB][HL];
+        double nq = 0.0, niq = 0.0, nQx = 0.0, nQy = 0.0, nQz = 0.0, nck = 0.0, nE[CLB][NZ];
         auto load_lane = [&](int c0) {
             const int c = c0 + lane;
             const bool live = c < nl;
@@ -332,8 +433,12 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
 #pragma unroll
             for (int j = 0; j < CLB; j++)
 #pragma unroll
-                for (int h = 0; h < HL; h++)
-                    nE[j][h] = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
+                for (int k = 0; k < NZ; k++) {
+                    if constexpr (SP)
+                        nE[j][k] = k < nzl[j] ? sd.ec[el[j] + (int64_t)k * capl[j] + cc] : 0.0;
+                    else
+                        nE[j][k] = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)k * capl[j] + cc] : 0.0;
+                }
         };
         if constexpr (PF) load_lane(0);
         for (int a = it.a0; a < it.a1; a++) {
@@ -349,7 +454,7 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 const int c = c0 + lane;
                 const bool live = c < nl;
                 const int cc = live ? c : c0; // in-range index for idle lanes
-                double q, iq, Qx, Qy, Qz, ck, Er[CLB][HL];
+                double q, iq, Qx, Qy, Qz, ck, Er[CLB][NZ];
                 if constexpr (PF) {
                     // this chunk's lane data were loaded one chunk ahead; the
                     // next chunk's (or, after the last, the first chunk of the
@@ -364,7 +469,7 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
 #pragma unroll
                     for (int j = 0; j < CLB; j++)
 #pragma unroll
-                        for (int h = 0; h < HL; h++) Er[j][h] = nE[j][h];
+                        for (int k = 0; k < NZ; k++) Er[j][k] = nE[j][k];
                     // (one chunk in all: its data stay in the prefetch set)
                     const int n0 = c0 + 64 < nl ? c0 + 64 : 0;
                     if (c0 + 64 < nl || (a + 1 < it.a1 && nl > 64)) load_lane(n0);
@@ -389,31 +494,49 @@ __global__ __launch_bounds__(64) void k_eri_shell_red(PP pp, SPD sd,
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
                 const double f = (cb * ck) * (((kPi52 * ip) * iq) * sqrt(is));
-                double El[CLB][HL];
+                // the lane E (dense: with the prefactor and the Hermite sign
+                // folded in here; SP: lane_contract applies both)
+                double El[CLB][NZ];
 #pragma unroll
                 for (int j = 0; j < CLB; j++)
 #pragma unroll
-                    for (int h = 0; h < HL; h++) {
-                        // (cl + j < it.cl1 always holds; as a condition it keeps
-                        // the compiler from issuing these loads one at a time
-                        // next to their uses, which serialized their latency)
-                        double e;
-                        if constexpr (PF)
-                            e = Er[j][h];
-                        else
-                            e = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)h * capl[j] + cc] : 0.0;
-                        El[j][h] = ((kH.t[h] + kH.u[h] + kH.v[h]) & 1) ? -(e * f) : e * f;
+                    for (int k = 0; k < NZ; k++) {
+                        // (guarded loads: as a condition the guard keeps the
+                        // compiler from issuing them one at a time next to
+                        // their uses, which serialized their latency; cl + j <
+                        // it.cl1 always holds)
+                        if constexpr (SP) {
+                            if constexpr (PF)
+                                El[j][k] = Er[j][k];
+                            else
+                                El[j][k] =
+                                    k < nzl[j] ? sd.ec[el[j] + (int64_t)k * capl[j] + cc] : 0.0;
+                        } else {
+                            double e;
+                            if constexpr (PF)
+                                e = Er[j][k];
+                            else
+                                e = cl + j < it.cl1 ? pp.E[el[j] + (int64_t)k * capl[j] + cc] : 0.0;
+                            El[j][k] = ((kH.t[k] + kH.u[k] + kH.v[k]) & 1) ? -(e * f) : e * f;
+                        }
                     }
+                if constexpr (SP) {
 #pragma unroll
-                for (int hl = 0; hl < HL; hl++)
+                    for (int j = 0; j < CLB; j++)
+                        lane_contract<LU, LL, false>(pat[j], El[j], R, Xa[j],
+                                                     [f](double e) { return e * f; });
+                } else {
 #pragma unroll
-                    for (int hu = 0; hu < HU; hu++)
+                    for (int hl = 0; hl < NZ; hl++)
 #pragma unroll
-                        for (int j = 0; j < CLB; j++)
-                            Xa[j][hu] = fma(El[j][hl],
-                                            R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
-                                                   kH.v[hu] + kH.v[hl])],
-                                            Xa[j][hu]);
+                        for (int hu = 0; hu < HU; hu++)
+#pragma unroll
+                            for (int j = 0; j < CLB; j++)
+                                Xa[j][hu] = fma(El[j][hl],
+                                                R[hidx(kH.t[hu] + kH.t[hl], kH.u[hu] + kH.u[hl],
+                                                       kH.v[hu] + kH.v[hl])],
+                                                Xa[j][hu]);
+                }
             }
             // the lane's uniform E, loaded here so that its latency hides
             // behind the LDS reduction below
@@ -498,20 +621,22 @@ __global__ void k_zero(double *x, int64_t n)
 // sums stay in the register file without spilling, and by shell-pair sizes:
 // an order-0 pair has 1 component, order 1 has 3 (p x s), order 2 has 9
 // (p x p) or 6 (d x s), so 3 covers them in as few passes as 4 does with
-// fewer registers.  (2, 2) in reduction mode runs passes of 2: with the
-// tabulated Boys function they fit 236 registers, two waves per SIMD, which
-// beats passes of 3 at one wave (949 against 1088 ms on the 200-AO molecule).
+// fewer registers.  (2, 2) in reduction mode runs passes of 2 at two waves
+// per SIMD, which beats passes of 3 at one wave (949 against 1088 ms on the
+// 200-AO molecule, dense lane E).  The wider passes of (2, 3), (2, 4),
+// (3, 3) and (4, 2) in reduction mode and of (3, 3) and (1, 4) in direct
+// mode pay off with the sparse lane E (kShellSparse) only.
 constexpr int kClbMax = 4;
 constexpr int kShellClb[5][5] = {{1, 3, 3, 4, 4},
-                                 {1, 3, 3, 4, 3},
+                                 {1, 3, 3, 4, 4},
                                  {1, 3, 3, 3, 1},
-                                 {1, 3, 3, 2, 1},
+                                 {1, 3, 3, 3, 1},
                                  {1, 3, 2, 1, 1}};
 constexpr int kShellClbRed[5][5] = {{1, 3, 3, 4, 2},
                                     {1, 3, 3, 3, 1},
-                                    {1, 3, 2, 2, 1},
-                                    {1, 3, 3, 1, 1},
-                                    {1, 2, 1, 1, 1}};
+                                    {1, 3, 2, 3, 2},
+                                    {1, 3, 3, 2, 1},
+                                    {1, 2, 2, 1, 1}};
 
 // Reduction-mode classes whose lane data are loaded one chunk ahead: those
 // at one wave per SIMD anyway, with room for a second set of lane registers
@@ -551,17 +676,43 @@ constexpr bool shell_prefetch_u(int LU, int LL, int B)
     return LU <= 1 && LL >= 3 && !(LL == 4 && LU + B >= 4);
 }
 
+// Classes that skip the structural zeros of the lane E (lane_contract), by
+// (uniform order, lane order): direct mode, reduction mode.  Chosen by
+// per-class kernel time on the 200-AO molecule: the pattern dispatch costs
+// scalar compares and registers, so it pays where the X contraction is
+// large against the Boys function and R table (uniform order >= 2 in
+// reduction mode) and not in the direct mode's short per-uniform-pair
+// iterations of uniform order <= 1 (lane order 3: 278 against 103 ms),
+// except (1, 4) with passes of 4 (about even); nor for (4, 4), whose 22
+// patterns of up to 25 coefficients spill (315 against 145 ms).
+constexpr bool kShellSparse[5][5] = {{false, false, false, false, false},
+                                     {false, false, false, false, true},
+                                     {false, false, false, false, false},
+                                     {false, false, false, true, false},
+                                     {false, false, false, false, false}};
+constexpr bool kShellSparseRed[5][5] = {{false, false, false, false, false},
+                                        {false, false, false, false, false},
+                                        {false, true, true, true, true},
+                                        {false, false, true, true, true},
+                                        {false, false, true, true, false}};
+constexpr bool shell_sparse(int LU, int LL, bool red)
+{
+    return red ? kShellSparseRed[LU][LL] : kShellSparse[LU][LL];
+}
+
 template <int LU, int LL, int B>
 void launch_shell_b(bool red, unsigned grid, hipStream_t st, const PP &pp, const SPD &sd,
                     const ShellItem *items, double *part)
 {
     if (red)
         hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B, shell_prefetch(LU, LL, B),
-                                            shell_boys_table(LU, LL, B, true)>),
+                                            shell_boys_table(LU, LL, B, true),
+                                            shell_sparse(LU, LL, true)>),
                            dim3(grid), dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu2, part);
     else
         hipLaunchKernelGGL((k_eri_shell<LU, LL, B, shell_prefetch_u(LU, LL, B),
-                                        shell_boys_table(LU, LL, B, false)>),
+                                        shell_boys_table(LU, LL, B, false),
+                                        shell_sparse(LU, LL, false)>),
                            dim3(grid), dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu, part);
 }
 
@@ -628,8 +779,17 @@ double boys_r_flops(int L)
     return 25.0 + 3.0 * (L + 1) + 4.0 * (HR4 - 1.0) + 12.0;
 }
 
-// X contraction and scaling per primitive quartet and lane component
-double x_flops(int LU, int LL) { return 2.0 * hc(LU) * hc(LL) + hc(LU); }
+// X contraction and scaling per primitive quartet and lane component with
+// nz nonzero Hermite coefficients
+double x_flops(int LU, double nz) { return 2.0 * hc(LU) * nz + hc(LU); }
+
+// nonzero Hermite coefficients of the lane components [l0, l1) of shell pair s
+int lane_nz(const zv_context *c, int s, int l0, int l1)
+{
+    int n = 0;
+    for (int q = c->h_sp_coff[s] + l0; q < c->h_sp_coff[s] + l1; q++) n += c->h_sp_cpat[q] >> 8;
+    return n;
+}
 
 // uniform components per item and the smallest count that uses the
 // reduction mode
@@ -678,10 +838,11 @@ inline int uniform_step(int ncu) { return ncu >= kRedMin ? kMaxURed : kMaxUDirec
 inline bool uniform_red(int ncu, int nu) { return ncu >= kRedMin && nu >= kRedMin; }
 
 // Kernel-time cost model of (U|L): lanes padded to 64, Boys and R redone per
-// pass of lane components and per uniform range.
-double pass_cost(int LU, int LL, int clb, int nu, bool red, double npu, double lanes)
+// pass of lane components and per uniform range; nz: mean nonzero Hermite
+// coefficients of a lane component.
+double pass_cost(int LU, int LL, int clb, int nu, bool red, double npu, double lanes, double nz)
 {
-    const double work = lanes * npu * (boys_r_flops(LU + LL) + clb * x_flops(LU, LL));
+    const double work = lanes * npu * (boys_r_flops(LU + LL) + clb * x_flops(LU, nz));
     return work + (red ? npu * finish_red(LU, clb) : lanes * npu * finish_direct(LU, clb, nu));
 }
 
@@ -691,14 +852,15 @@ double item_cost(const zv_context *c, int su, int sl)
     const int ncu = c->h_sp_coff[su + 1] - c->h_sp_coff[su];
     const int ncl = c->h_sp_coff[sl + 1] - c->h_sp_coff[sl];
     const double npu = c->h_sp_np[su], lanes = std::ceil(c->h_sp_np[sl] / 64.0) * 64.0;
+    const double nz = hc(LL);
     const int step = uniform_step(ncu);
     double cost = 0.0;
     for (int cu0 = 0; cu0 < ncu; cu0 += step) {
         const int nu = std::min(step, ncu - cu0);
         const bool red = uniform_red(ncu, nu);
         const int bm = lane_batch(LU, LL, red), n2 = full_passes_end(LU, LL, ncl, red);
-        cost += n2 / bm * pass_cost(LU, LL, bm, nu, red, npu, lanes);
-        if (ncl > n2) cost += pass_cost(LU, LL, ncl - n2, nu, red, npu, lanes);
+        cost += n2 / bm * pass_cost(LU, LL, bm, nu, red, npu, lanes, nz);
+        if (ncl > n2) cost += pass_cost(LU, LL, ncl - n2, nu, red, npu, lanes, nz);
     }
     return cost;
 }
@@ -756,6 +918,7 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
         const double lanes = std::ceil(npl / 64.0) * 64.0;
         const int k = 5 * LU + LL;
         const double HU = hc(LU);
+        const double nz = hc(LL);
         const int step = uniform_step(ncu);
         bool br_counted = false; // Boys and R counted once per shell quartet
         for (int cu0 = 0; cu0 < ncu; cu0 += step) {
@@ -769,7 +932,7 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
                 if (clb <= 0) continue;
                 const int npass = (s1 - s0) / clb;
                 if (npass <= 0) continue;
-                const double wp = pass_cost(LU, LL, clb, nu, red, npu, lanes); // per pass
+                const double wp = pass_cost(LU, LL, clb, nu, red, npu, lanes, nz); // per pass
                 const int k1 = (int)std::min<double>(
                     npass, std::max(1.0, std::ceil(npass * wp / budget)));
                 const int b = 25 * (2 * (clb - 1) + (red ? 1 : 0)) + k;
@@ -784,12 +947,15 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
                     pp.groups.push_back(ShellGroup{o.su, o.sl, cu0, nu, l0, nlr, ns, 0, pp.npart});
                     // algorithmic flops: Boys and R once per primitive quartet
                     // (however many lane passes redo them); per lane component
-                    // the X contraction, then either nu direct dot products per
-                    // primitive quartet or the sum over lane primitives per
-                    // primitive quartet plus nu dot products per uniform primitive
+                    // the X contraction over its nonzero Hermite coefficients,
+                    // then either nu direct dot products per primitive quartet
+                    // or the sum over lane primitives per primitive quartet
+                    // plus nu dot products per uniform primitive
                     const double pq = (double)npu * npl;
                     pp.units[b] += pq * nlr;
-                    double fl = pq * ((br_counted ? 0.0 : boys_r_flops(L)) + nlr * x_flops(LU, LL));
+                    const double xf =
+                        2.0 * HU * lane_nz(c, o.sl, l0, l1) + nlr * HU; // sum of x_flops
+                    double fl = pq * ((br_counted ? 0.0 : boys_r_flops(L)) + xf);
                     br_counted = true;
                     fl += red ? nlr * (pq * HU + (double)npu * 2.0 * nu * HU)
                               : nlr * pq * 2.0 * nu * HU;
@@ -914,9 +1080,28 @@ static SPD spd_of(zv_context *c);
 int zv_shell_ready(zv_context *c, const PP &pp)
 {
     const int nsp = c->nsp;
-    size_t meta = sizeof(int) * ((size_t)c->npair + nsp + 1) + sizeof(int64_t) * 2 * nsp + 64;
+    size_t meta =
+        sizeof(int) * (2 * (size_t)c->npair + nsp + 1) + sizeof(int64_t) * 2 * nsp + 64;
     ZV_CHECK(zv_ensure(c, c->b_spmeta, meta));
     if (c->sp_ready) return ZV_OK;
+    // Hermite pattern of each component with a shell-kernel order
+    c->h_sp_cpat.assign(c->npair, 0);
+    for (int q = 0; q < c->npair; q++) {
+        const int P = c->h_sp_comp[q];
+        const int L = c->h_pair_L[P];
+        if (L > ZV_LMAX_T) continue;
+        const int i = c->h_pair_i[P] - 1, k = c->h_pair_k[P] - 1;
+        int S[3], deg = 0;
+        for (int d = 0; d < 3; d++) {
+            S[d] = c->h_npower[3 * i + d] + c->h_npower[3 * k + d];
+            deg += S[d];
+        }
+        const int nr = (c->h_itype[i] == 2) + (c->h_itype[k] == 2);
+        const int pid = deg + 2 * nr == L ? kHP.find(L, herm_union(S[0], S[1], S[2], nr)) : -1;
+        if (pid < 0)
+            return zv_fail(c, ZV_ERR_STATE, "AO pair %d: no Hermite pattern of order %d", P, L);
+        c->h_sp_cpat[q] = pid | kHP.nz[L][pid] << 8;
+    }
     std::vector<int64_t> puoff(nsp), euoff(nsp);
     c->h_sp_np.assign(nsp, 0);
     int64_t pu_tot = 0, eu_tot = 0;
@@ -932,11 +1117,13 @@ int zv_shell_ready(zv_context *c, const PP &pp)
     ZV_CHECK(zv_ensure(c, c->b_spu, sizeof(double) * std::max<int64_t>(pu_tot, 1)));
     // two layouts of the uniform-side E: [a][comp][h] then [a][h][comp]
     ZV_CHECK(zv_ensure(c, c->b_spe, sizeof(double) * std::max<int64_t>(2 * eu_tot, 1)));
+    ZV_CHECK(zv_ensure(c, c->b_spec, sizeof(double) * std::max<int64_t>(c->e_total, 1)));
     c->sp_eu_total = eu_tot;
     int64_t *d_puoff = (int64_t *)c->b_spmeta.p;
     int64_t *d_euoff = d_puoff + nsp;
     int *d_comp = (int *)(d_euoff + nsp);
     int *d_coff = d_comp + c->npair;
+    int *d_cpat = d_coff + nsp + 1;
     ZV_HIPCHECK(c, hipMemcpyAsync(d_puoff, puoff.data(), sizeof(int64_t) * nsp,
                                   hipMemcpyHostToDevice, c->stream));
     ZV_HIPCHECK(c, hipMemcpyAsync(d_euoff, euoff.data(), sizeof(int64_t) * nsp,
@@ -945,12 +1132,17 @@ int zv_shell_ready(zv_context *c, const PP &pp)
                                   hipMemcpyHostToDevice, c->stream));
     ZV_HIPCHECK(c, hipMemcpyAsync(d_coff, c->h_sp_coff.data(), sizeof(int) * (nsp + 1),
                                   hipMemcpyHostToDevice, c->stream));
+    ZV_HIPCHECK(c, hipMemcpyAsync(d_cpat, c->h_sp_cpat.data(), sizeof(int) * c->npair,
+                                  hipMemcpyHostToDevice, c->stream));
     SPD sd = spd_of(c);
     {
         PhaseTimer tm(c, PH_PRIM);
         hipLaunchKernelGGL(k_shellpairs, dim3(nsp), dim3(256), 0, c->stream, pp, sd,
                            (double *)c->b_spu.p, (double *)c->b_spe.p,
                            (double *)c->b_spe.p + eu_tot);
+        ZV_HIPCHECK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_compact_e, dim3(c->npair), dim3(256), 0, c->stream, pp, sd,
+                           (double *)c->b_spec.p);
         ZV_HIPCHECK(c, hipGetLastError());
     }
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -964,9 +1156,10 @@ static SPD spd_of(zv_context *c)
     int64_t *d_euoff = d_puoff + c->nsp;
     int *d_comp = (int *)(d_euoff + c->nsp);
     int *d_coff = d_comp + c->npair;
+    const int *d_cpat = d_coff + c->nsp + 1;
     const double *eu = (const double *)c->b_spe.p;
     return SPD{d_comp, d_coff, d_puoff, d_euoff, (const double *)c->b_spu.p, eu,
-               eu + c->sp_eu_total};
+               eu + c->sp_eu_total, d_cpat, (const double *)c->b_spec.p};
 }
 
 // Plan and launch the shell kernels of one Schwarz phase over the shell

@@ -274,7 +274,7 @@ extern "C" int zv_destroy(zv_context *c)
     DevBuf *bufs[] = {&c->b_r1, &c->b_r2, &c->b_pi0, &c->b_pd, &c->b_accS, &c->b_accG,
                       &c->b_s1, &c->b_s2, &c->b_out, &c->b_out2, &c->b_blk, &c->b_pp, &c->b_btab,
                       &c->b_ppE, &c->b_ppcnt, &c->b_tri, &c->b_list, &c->b_part, &c->b_misc,
-                      &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_sitem1, &c->b_sgroup1,
+                      &c->b_spmeta, &c->b_spu, &c->b_spe, &c->b_spec, &c->b_sitem1, &c->b_sgroup1,
                       &c->b_part1, &c->b_sitem2, &c->b_sgroup2, &c->b_part2, &c->b_plist,
                       &c->b_gmeta, &c->b_res[0], &c->b_res[1], &c->b_res[2], &c->b_mpart};
     for (DevBuf *b : bufs)

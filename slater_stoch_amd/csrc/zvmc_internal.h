@@ -94,9 +94,11 @@ struct zv_context {
     DevBuf b_misc;
     bool pp_ready = false;
     // shell-pair data: component tables and the uniform-side copies
-    DevBuf b_spmeta;                   // puoff[nsp], euoff[nsp], comp[npair], coff[nsp+1]
+    DevBuf b_spmeta;                   // puoff[nsp], euoff[nsp], comp[npair], coff[nsp+1], cpat[npair]
     DevBuf b_spu, b_spe;               // per primitive pair: p,1/p,P,c; E [a][comp][h]
+    DevBuf b_spec;                     // lane-side E without structural zeros (k_compact_e)
     std::vector<int> h_sp_np;          // kept primitive pairs per shell pair
+    std::vector<int> h_sp_cpat;        // per component: Hermite pattern | nonzero count << 8
     int64_t sp_eu_total = 0;           // doubles of one uniform-side E layout
     bool sp_ready = false;
     // group-blocked estimator: units of 1, 3 or 6 components of one shell

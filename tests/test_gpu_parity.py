@@ -224,6 +224,24 @@ def test_gauss_eri_full_list_equals_explicit_list(engine):
     np.testing.assert_allclose(c, a, rtol=1e-12, atol=1e-16)
 
 
+def test_gauss_eri_every_quartet_of_all_orbital_types(engine):
+    """Every quartet of the mixed basis (s, r^2-type s, p, r^2-type p, d, f, g
+    AOs): the shell kernels, which skip the structural zeros of each lane
+    pair's Hermite coefficients (a box of per-axis power sums, a union of
+    boxes for r^2-type AOs), against the component kernels, which sum over
+    every coefficient.  A pattern missing a nonzero coefficient would drop a
+    term here."""
+    bas = _mixed_basis()
+    engine.set_basis(bas)
+    eri, pre, _ = engine.gauss_eri()
+    assert any(k[0] == "shell" for k in engine.eri_stats())
+    i, j, k, l = B.quartet_mapping(bas.n)
+    keep = np.abs(pre[k - 1, i - 1] * pre[l - 1, j - 1]) >= 1e-30
+    a = engine.gauss_ijkl(i[keep], k[keep], j[keep], l[keep])
+    scale = np.abs(a).max()
+    np.testing.assert_allclose(eri[keep], a, rtol=1e-12, atol=1e-15 * scale)
+
+
 def _kcp_lookup(is_, js, ks, ls, n):
     """Reference position of quartet (i k | j l) (any of its 8 equivalent forms)."""
     key = lambda a, b, c, d: ((a.astype(np.int64) * (n + 1) + b) * (n + 1) + c) * (n + 1) + d
