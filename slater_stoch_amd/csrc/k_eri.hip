@@ -30,9 +30,11 @@
 // bra primitive pair is wave-uniform, the 64 lanes stride over the ket
 // primitive pairs.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <thread>
 
 #include "eri_device.h"
 #include "zvmc_internal.h"
@@ -633,40 +635,79 @@ int zv_run_gauss_eri(zv_context *c, double *ijkl, double *precond, int64_t *n_sc
                                   c->stream));
     ZV_HIPCHECK(c, hipStreamSynchronize(c->stream));
     // phase 2
+    const auto tp2 = std::chrono::steady_clock::now();
     std::vector<double> spmax(nsp, 0.0);
     for (int s = 0; s < nsp; s++) {
         auto r = comps(s);
         for (const int *P = r.first; P != r.second; P++)
             spmax[s] = std::max(spmax[s], fabs(hpre[*P]));
     }
+    // the lists of shell pairs x in [x0, x1), on host threads over ranges of
+    // x of equal work (~x each), joined in order: the same lists as one pass
+    struct Lists {
+        std::vector<std::pair<int, int>> sq;
+        std::vector<int> gb, gk;
+    };
+    auto build = [&](int x0, int x1, Lists &o) {
+        for (int x = x0; x < x1; x++) {
+            if (!c->h_sp_np[x]) continue;
+            auto rx = comps(x);
+            if (!templ(x))
+                for (const int *P = rx.first; P != rx.second; P++)
+                    for (const int *Q = rx.first; Q != rx.second; Q++)
+                        if (*P > *Q && fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
+                            o.gb.push_back(*P);
+                            o.gk.push_back(*Q);
+                        }
+            for (int y = 0; y < x; y++) {
+                if (!c->h_sp_np[y] || spmax[x] * spmax[y] < 1.0e-30) continue;
+                if (templ(x) && templ(y)) {
+                    o.sq.push_back({x, y});
+                    continue;
+                }
+                auto ry = comps(y);
+                for (const int *P = rx.first; P != rx.second; P++)
+                    for (const int *Q = ry.first; Q != ry.second; Q++)
+                        if (fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
+                            o.gb.push_back(*P);
+                            o.gk.push_back(*Q);
+                        }
+            }
+        }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = nsp < 512 ? 1 : (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    std::vector<Lists> lists(nt);
+    std::vector<int> xb(nt + 1);
+    for (int t = 0; t <= nt; t++) xb[t] = (int)std::lround(nsp * std::sqrt((double)t / nt));
+    xb[nt] = nsp;
+    {
+        std::vector<std::thread> th;
+        int started = 1;
+        for (int t = 1; t < nt; t++, started++) {
+            try {
+                th.emplace_back(build, xb[t], xb[t + 1], std::ref(lists[t]));
+            } catch (...) {
+                break;
+            }
+        }
+        for (int t = started; t < nt; t++) build(xb[t], xb[t + 1], lists[t]);
+        build(xb[0], xb[1], lists[0]);
+        for (auto &x : th) x.join();
+    }
     sq.clear();
     gb.clear();
     gk.clear();
-    for (int x = 0; x < nsp; x++) {
-        if (!c->h_sp_np[x]) continue;
-        auto rx = comps(x);
-        if (!templ(x))
-            for (const int *P = rx.first; P != rx.second; P++)
-                for (const int *Q = rx.first; Q != rx.second; Q++)
-                    if (*P > *Q && fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
-                        gb.push_back(*P);
-                        gk.push_back(*Q);
-                    }
-        for (int y = 0; y < x; y++) {
-            if (!c->h_sp_np[y] || spmax[x] * spmax[y] < 1.0e-30) continue;
-            if (templ(x) && templ(y)) {
-                sq.push_back({x, y});
-                continue;
-            }
-            auto ry = comps(y);
-            for (const int *P = rx.first; P != rx.second; P++)
-                for (const int *Q = ry.first; Q != ry.second; Q++)
-                    if (fabs(hpre[*P] * hpre[*Q]) >= 1.0e-30) {
-                        gb.push_back(*P);
-                        gk.push_back(*Q);
-                    }
-        }
+    for (const Lists &o : lists) {
+        sq.insert(sq.end(), o.sq.begin(), o.sq.end());
+        gb.insert(gb.end(), o.gb.begin(), o.gb.end());
+        gk.insert(gk.end(), o.gk.begin(), o.gk.end());
     }
+    if (c->verbose)
+        fprintf(stderr, "[zvmc] Schwarz phase 2 list: %zu shell quartets, %zu component quartets, "
+                "%.1f ms\n", sq.size(), gb.size(),
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp2)
+                    .count());
     ZV_CHECK(zv_shell_run(c, pp, sq, c->b_sitem2, c->b_sgroup2, c->b_part2, c->h_plan2, &ng2));
     ZV_CHECK(zv_shell_scatter(c, c->b_sgroup2, c->b_part2, ng2, pre, tri));
     // the off-diagonal components of (x|x) get the test now that precond is known

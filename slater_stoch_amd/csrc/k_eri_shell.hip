@@ -30,6 +30,7 @@
 // X contraction over them only, the same terms in the same order minus
 // exact zeros.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -784,11 +785,9 @@ double boys_r_flops(int L)
 double x_flops(int LU, double nz) { return 2.0 * hc(LU) * nz + hc(LU); }
 
 // nonzero Hermite coefficients of the lane components [l0, l1) of shell pair s
-int lane_nz(const zv_context *c, int s, int l0, int l1)
+int64_t lane_nz(const zv_context *c, int s, int l0, int l1)
 {
-    int n = 0;
-    for (int q = c->h_sp_coff[s] + l0; q < c->h_sp_coff[s] + l1; q++) n += c->h_sp_cpat[q] >> 8;
-    return n;
+    return c->h_sp_nzcum[c->h_sp_coff[s] + l1] - c->h_sp_nzcum[c->h_sp_coff[s] + l0];
 }
 
 // uniform components per item and the smallest count that uses the
@@ -810,13 +809,19 @@ double finish_red(int LU, int clb)
 // bucket b = 25 (2 (clb - 1) + red) + 5 LU + LL
 constexpr int kNB = 25 * 2 * kClbMax;
 
+// A plan's items (bucket b at [lo[b], lo[b + 1])) and groups live in the
+// caller's host buffer (items, then groups), from which they are copied to
+// the device as they are.
 struct Plan {
-    // bucket b at [lo[b], lo[b + 1])
-    std::vector<ShellItem> items;
+    ShellItem *items = nullptr;
+    size_t nitems = 0;
     int lo[kNB + 1] = {};
-    std::vector<ShellGroup> groups;
+    ShellGroup *groups = nullptr;
+    size_t ngroups = 0;
     int64_t npart = 0;
     double units[kNB] = {}, flops[kNB] = {}; // per bucket
+    double host_ms[4] = {};                  // orientation, items, join, bucket sort
+    bool bad = false;                        // an item the kernels cannot take
 };
 
 // lane components [0, full) run in passes of the class's batch limit, the
@@ -846,12 +851,8 @@ double pass_cost(int LU, int LL, int clb, int nu, bool red, double npu, double l
     return work + (red ? npu * finish_red(LU, clb) : lanes * npu * finish_direct(LU, clb, nu));
 }
 
-double item_cost(const zv_context *c, int su, int sl)
+double item_cost_of(int LU, int LL, int ncu, int ncl, double npu, double lanes)
 {
-    const int LU = c->h_sp_L[su], LL = c->h_sp_L[sl];
-    const int ncu = c->h_sp_coff[su + 1] - c->h_sp_coff[su];
-    const int ncl = c->h_sp_coff[sl + 1] - c->h_sp_coff[sl];
-    const double npu = c->h_sp_np[su], lanes = std::ceil(c->h_sp_np[sl] / 64.0) * 64.0;
     const double nz = hc(LL);
     const int step = uniform_step(ncu);
     double cost = 0.0;
@@ -864,6 +865,51 @@ double item_cost(const zv_context *c, int su, int sl)
     }
     return cost;
 }
+
+// The planned cost of an orientation is npu (lanes A + B) with A and B fixed
+// by the (order, components) types of the uniform and the lane shell pair
+// (pass_cost is linear in npu and lanes), tabulated once per plan.
+struct CostTable {
+    std::vector<int> type;        // per shell pair
+    std::vector<double> npu, lanes; // per shell pair
+    std::vector<double> A, B;     // [type u][type l]
+    int nt = 0;
+    explicit CostTable(const zv_context *c)
+    {
+        const int nsp = c->nsp;
+        std::vector<std::pair<int, int>> keys;
+        type.resize(nsp);
+        npu.resize(nsp);
+        lanes.resize(nsp);
+        for (int s = 0; s < nsp; s++) {
+            const std::pair<int, int> k{c->h_sp_L[s], c->h_sp_coff[s + 1] - c->h_sp_coff[s]};
+            auto it = std::find(keys.begin(), keys.end(), k);
+            type[s] = (int)(it - keys.begin());
+            if (it == keys.end()) keys.push_back(k);
+            npu[s] = c->h_sp_np[s];
+            lanes[s] = std::ceil(c->h_sp_np[s] / 64.0) * 64.0;
+        }
+        nt = (int)keys.size();
+        A.resize((size_t)nt * nt);
+        B.resize((size_t)nt * nt);
+        for (int u = 0; u < nt; u++)
+            for (int l = 0; l < nt; l++) {
+                const auto [LU, ncu] = keys[u];
+                const auto [LL, ncl] = keys[l];
+                // shell quartets of these orders only (the others go through
+                // component lists and have no shell kernel)
+                if (LU > ZV_LMAX_T || LL > ZV_LMAX_T) continue;
+                const double b = item_cost_of(LU, LL, ncu, ncl, 1.0, 0.0);
+                A[(size_t)u * nt + l] = item_cost_of(LU, LL, ncu, ncl, 1.0, 1.0) - b;
+                B[(size_t)u * nt + l] = b;
+            }
+    }
+    double cost(int su, int sl) const
+    {
+        const size_t k = (size_t)type[su] * nt + type[sl];
+        return npu[su] * (lanes[sl] * A[k] + B[k]);
+    }
+};
 
 // Host threads for the planner: up to 16 (the host share of one GPU), one
 // for short lists.
@@ -954,7 +1000,7 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
                     const double pq = (double)npu * npl;
                     pp.units[b] += pq * nlr;
                     const double xf =
-                        2.0 * HU * lane_nz(c, o.sl, l0, l1) + nlr * HU; // sum of x_flops
+                        2.0 * HU * (double)lane_nz(c, o.sl, l0, l1) + nlr * HU; // sum of x_flops
                     double fl = pq * ((br_counted ? 0.0 : boys_r_flops(L)) + xf);
                     br_counted = true;
                     fl += red ? nlr * (pq * HU + (double)npu * 2.0 * nu * HU)
@@ -980,15 +1026,30 @@ void plan_items(const zv_context *c, const Orient *ors, size_t n, double budget,
 // uniform primitive-pair slices (partial sums added by the scatter).  Long
 // lists are planned on host threads over contiguous runs of quartets whose
 // items are joined in order, so the plan is the same for any thread count.
-void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq, Plan &pl)
+// the kernels assume whole lane passes (no guard per lane component) and,
+// in direct mode, at most kDirectMaxU uniform components
+inline bool item_ok(const ShellItem &it, int clb, bool red)
 {
+    return it.cl1 > it.cl0 && (it.cl1 - it.cl0) % clb == 0 &&
+           (red || (it.nu >= 1 && it.nu <= kDirectMaxU));
+}
+
+void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq, Plan &pl,
+                std::vector<char> &hbuf)
+{
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) {
+        return std::chrono::duration<double, std::milli>(clk::now() - t).count();
+    };
+    auto tt = clk::now();
     const size_t nq = sq.size();
     const int nt = plan_threads(nq);
+    const CostTable ct(c);
     std::vector<Orient> ors(nq);
     run_threads(nt, [&](int t) {
         for (size_t q = nq * t / nt; q < nq * (t + 1) / nt; q++) {
             const int x = sq[q].first, y = sq[q].second;
-            double cxy = item_cost(c, x, y), cyx = x == y ? cxy : item_cost(c, y, x);
+            double cxy = ct.cost(x, y), cyx = x == y ? cxy : ct.cost(y, x);
             ors[q] = cxy <= cyx ? Orient{x, y, cxy} : Orient{y, x, cyx};
         }
     });
@@ -999,11 +1060,15 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
     const double nitems =
         c->tune[ZV_TUNE_ERI_ITEMS] > 0 ? (double)c->tune[ZV_TUNE_ERI_ITEMS] : 40000.0;
     const double budget = std::max(total / nitems, 2.0e6);
+    pl.host_ms[0] = ms_since(tt);
+    tt = clk::now();
     std::vector<PlanPart> parts(nt);
     run_threads(nt, [&](int t) {
         const size_t q0 = nq * t / nt, q1 = nq * (t + 1) / nt;
         plan_items(c, ors.data() + q0, q1 - q0, budget, parts[t]);
     });
+    pl.host_ms[1] = ms_since(tt);
+    tt = clk::now();
     // join the runs in order: partial-sum offsets shifted by the runs before
     pl.npart = 0;
     size_t ng = 0, nb_items[kNB] = {};
@@ -1011,33 +1076,45 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         ng += pp.groups.size();
         for (int b = 0; b < kNB; b++) nb_items[b] += pp.tmp[b].size();
     }
-    pl.groups.reserve(ng);
     size_t at = 0;
     for (int b = 0; b < kNB; b++) {
         pl.lo[b] = (int)at;
         at += nb_items[b];
     }
     pl.lo[kNB] = (int)at;
-    pl.items.resize(at);
+    hbuf.resize(sizeof(ShellItem) * at + sizeof(ShellGroup) * ng);
+    pl.items = (ShellItem *)hbuf.data();
+    pl.nitems = at;
+    pl.groups = (ShellGroup *)(hbuf.data() + sizeof(ShellItem) * at);
+    pl.ngroups = ng;
     std::vector<int64_t> off(nt, 0);
-    for (int t = 0; t < nt; t++) {
+    std::vector<size_t> goff(nt, 0);
+    for (int t = 0, g0 = 0; t < nt; t++) {
         PlanPart &pp = parts[t];
         off[t] = pl.npart;
-        for (ShellGroup g : pp.groups) {
-            g.part += off[t];
-            pl.groups.push_back(g);
-        }
+        goff[t] = (size_t)g0;
+        g0 += (int)pp.groups.size();
         for (int b = 0; b < kNB; b++) {
             pl.units[b] += pp.units[b];
             pl.flops[b] += pp.flops[b];
         }
         pl.npart += pp.npart;
     }
+    run_threads(nt, [&](int t) {
+        ShellGroup *dst = pl.groups + goff[t];
+        for (ShellGroup g : parts[t].groups) {
+            g.part += off[t];
+            *dst++ = g;
+        }
+    });
+    pl.host_ms[2] = ms_since(tt);
+    tt = clk::now();
     // each bucket in descending cost (quarter-octave bins, counting sort), so the
     // long items start first and the queue ends on short ones; buckets are
     // independent and spread over the threads
     constexpr int NBIN = 256;
     static_assert(NBIN <= 256, "bins are stored as bytes");
+    std::vector<char> bad(nt, 0);
     run_threads(nt, [&](int t) {
         std::vector<uint8_t> bin;
         std::vector<const ShellItem *> src;
@@ -1045,6 +1122,8 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
         for (int b = t; b < kNB; b += nt) {
             const size_t n = nb_items[b];
             if (!n) continue;
+            const int clb = b / 50 + 1;
+            const bool red = ((b / 25) & 1) != 0;
             src.clear();
             cst.clear();
             std::vector<int64_t> shift;
@@ -1065,10 +1144,13 @@ void plan_shell(const zv_context *c, const std::vector<std::pair<int, int>> &sq,
             for (size_t i = 0; i < n; i++) {
                 ShellItem it = *src[i];
                 it.part += shift[i];
+                if (!item_ok(it, clb, red)) bad[t] = 1;
                 pl.items[pl.lo[b] + cnt[bin[i]]++] = it;
             }
         }
     });
+    for (char x : bad) pl.bad = pl.bad || x;
+    pl.host_ms[3] = ms_since(tt);
 }
 
 } // namespace
@@ -1102,6 +1184,9 @@ int zv_shell_ready(zv_context *c, const PP &pp)
             return zv_fail(c, ZV_ERR_STATE, "AO pair %d: no Hermite pattern of order %d", P, L);
         c->h_sp_cpat[q] = pid | kHP.nz[L][pid] << 8;
     }
+    c->h_sp_nzcum.assign(c->npair + 1, 0);
+    for (int q = 0; q < c->npair; q++)
+        c->h_sp_nzcum[q + 1] = c->h_sp_nzcum[q] + (c->h_sp_cpat[q] >> 8);
     std::vector<int64_t> puoff(nsp), euoff(nsp);
     c->h_sp_np.assign(nsp, 0);
     int64_t pu_tot = 0, eu_tot = 0;
@@ -1168,21 +1253,41 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
                  DevBuf &bitem, DevBuf &bgroup, DevBuf &bpart, std::vector<char> &hplan,
                  int64_t *ngroups)
 {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     Plan pl;
-    plan_shell(c, sq, pl);
-    *ngroups = (int64_t)pl.groups.size();
-    if (pl.items.empty()) return ZV_OK;
-    const size_t bi = sizeof(ShellItem) * pl.items.size();
-    const size_t bg = sizeof(ShellGroup) * pl.groups.size();
-    hplan.resize(bi + bg);
-    std::memcpy(hplan.data(), pl.items.data(), bi);
-    std::memcpy(hplan.data() + bi, pl.groups.data(), bg);
+    plan_shell(c, sq, pl, hplan);
+    const auto t1 = clk::now();
+    *ngroups = (int64_t)pl.ngroups;
+    if (!pl.nitems) return ZV_OK;
+    if (pl.bad) {
+        // the first offending item, for the message
+        for (int b = 0; b < kNB; b++)
+            for (int t = pl.lo[b]; t < pl.lo[b + 1]; t++) {
+                const ShellItem &it = pl.items[t];
+                if (!item_ok(it, b / 50 + 1, ((b / 25) & 1) != 0))
+                    return zv_fail(c, ZV_ERR_STATE, "shell item with lane components [%d, %d) "
+                                   "and %d uniform components in bucket %d", it.cl0, it.cl1,
+                                   it.nu, b);
+            }
+    }
+    const size_t bi = sizeof(ShellItem) * pl.nitems;
+    const size_t bg = sizeof(ShellGroup) * pl.ngroups;
+    const auto t2 = clk::now();
     ZV_CHECK(zv_ensure(c, bitem, bi));
     ZV_CHECK(zv_ensure(c, bgroup, bg));
     ZV_CHECK(zv_ensure(c, bpart, sizeof(double) * pl.npart));
     ZV_HIPCHECK(c, hipMemcpyAsync(bitem.p, hplan.data(), bi, hipMemcpyHostToDevice, c->stream));
     ZV_HIPCHECK(c, hipMemcpyAsync(bgroup.p, hplan.data() + bi, bg, hipMemcpyHostToDevice,
                                   c->stream));
+    if (c->verbose) {
+        const auto t3 = clk::now();
+        auto ms = [](clk::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+        fprintf(stderr, "[zvmc] shell plan: %zu quartets -> %zu items, %zu groups; plan %.1f ms "
+                "(orientation %.1f, items %.1f, join %.1f, bucket sort %.1f), device alloc + "
+                "copy %.1f ms\n", sq.size(), pl.nitems, pl.ngroups, ms(t1 - t0), pl.host_ms[0],
+                pl.host_ms[1], pl.host_ms[2], pl.host_ms[3], ms(t3 - t2));
+    }
     const SPD sd = spd_of(c);
     const ShellItem *di = (const ShellItem *)bitem.p;
     // The buckets write disjoint partial sums, so they run side by side on
@@ -1192,21 +1297,6 @@ int zv_shell_run(zv_context *c, const PP &pp, const std::vector<std::pair<int, i
     // cost so far.
     // bucket b = 25 v + 5 LU + LL, v = 2 (clb - 1) + red: statistics slot 36 + b
     static_assert(36 + kNB <= ZV_NSTAT, "statistics slots");
-    // the kernels assume whole lane passes (no guard per lane component) and,
-    // in direct mode, at most kDirectMaxU uniform components
-    for (int b = 0; b < kNB; b++) {
-        const int clb = b / 50 + 1;
-        const bool red = ((b / 25) & 1) != 0;
-        for (int t = pl.lo[b]; t < pl.lo[b + 1]; t++) {
-            const ShellItem &it = pl.items[t];
-            if (it.cl1 <= it.cl0 || (it.cl1 - it.cl0) % clb != 0)
-                return zv_fail(c, ZV_ERR_STATE, "shell item with lane components [%d, %d) "
-                               "in passes of %d", it.cl0, it.cl1, clb);
-            if (!red && (it.nu < 1 || it.nu > kDirectMaxU))
-                return zv_fail(c, ZV_ERR_STATE, "direct shell item with %d uniform components",
-                               it.nu);
-        }
-    }
     constexpr int NST = 1 + zv_context::NSIDE;
     hipStream_t sts[NST] = {c->stream};
     for (int k = 0; k < zv_context::NSIDE; k++) sts[1 + k] = c->side[k];

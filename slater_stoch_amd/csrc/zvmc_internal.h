@@ -99,6 +99,7 @@ struct zv_context {
     DevBuf b_spec;                     // lane-side E without structural zeros (k_compact_e)
     std::vector<int> h_sp_np;          // kept primitive pairs per shell pair
     std::vector<int> h_sp_cpat;        // per component: Hermite pattern | nonzero count << 8
+    std::vector<int64_t> h_sp_nzcum;   // running sum of the components' nonzero counts
     int64_t sp_eu_total = 0;           // doubles of one uniform-side E layout
     bool sp_ready = false;
     // group-blocked estimator: units of 1, 3 or 6 components of one shell
