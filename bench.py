@@ -8,7 +8,7 @@ exponents (example/VB1), 20100 pairs, 202,015,050 quartets, 20 Gaussians per
 STO fit:
 
   1. analytic Gaussian-fit ERIs with the Cauchy-Schwarz driver (FP64),
-  2. a pilot of 10 blocks x 4000 Monte Carlo samples over every quartet,
+  2. a pilot of 10 blocks x 1000 Monte Carlo samples over every quartet,
   3. while the largest standard error is above 1e-6 Ha: a further run of 10
      blocks for the quartets above it only (zv_mc_refine: the estimator
      tiles that hold them), sized from their error (stderr ~ 1/sqrt(samples)).
@@ -45,7 +45,7 @@ from slater_stoch_amd.shells import kernel_unit_pairs, mfma_tile_use, shell_firs
 METRIC = "sec to 1e-6 Ha stderr on all (ij|kl), mixed-prec vs FP64 GPU vs CPU; MFMA %"
 TARGET = 1e-6
 NBL = 10
-PILOT = 4000
+PILOT = 1000  # samples per pilot block
 MAX_ROUNDS = 6
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32
@@ -130,7 +130,7 @@ def precision_report(bas, m64, e64, m32):
 
 
 def same_sample_precision(eng, bas, prec):
-    """prec against FP64 on the same 10 x 4000 samples (a pilot run)."""
+    """prec against FP64 on the same 10 x PILOT samples (a pilot run)."""
     eng.set_basis(bas)
     m64, e64 = eng.mc_blocks(NBL, PILOT, "fp64", seed=3, stream=0)
     m32, _ = eng.mc_blocks(NBL, PILOT, prec, seed=3, stream=0)
@@ -334,7 +334,7 @@ def cpu_baseline(bas, pre, nscr, pilot_emax, budget_s=24.0):
     reference's compute_densities) for one 40-sample walker block over two
     random quartet subsets, fitted as a + b nq and scaled to every quartet
     and to the samples the reference's own policy needs (one npts for all
-    quartets: the pilot's 4000 per block times (pilot max stderr / 1e-6)^2).
+    quartets: the pilot's samples per block times (pilot max stderr / 1e-6)^2).
     The oracle's C port (OpenMP, all host cores) is timed on the same
     quartets beside it (``port``): the calibration ratio port / reference."""
     sys.path.insert(0, str(ROOT / "tests"))

@@ -66,7 +66,7 @@ def next_npts(npts, err_max, target, safety, cap_npts):
     return min(want, cap) if cap > npts else 0
 
 
-def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=4000,
+def run_molecule(eng, bas, target=1e-6, precision="fp32", nbl=10, pilot_npts=1000,
                  max_samples=10_000_000, safety=1.2, max_rounds=4, seed=0, keep_arrays=False,
                  policy="refine"):
     """One molecule: analytic ERIs, pilot, then further runs by the policy
@@ -186,6 +186,8 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=1,
                     help="must be 1: concurrent contexts are not supported")
     ap.add_argument("--max-samples", type=int, default=10_000_000)
+    ap.add_argument("--pilot-npts", type=int, default=1000,
+                    help="samples per pilot block (a multiple of 40)")
     ap.add_argument("--max-rounds", type=int, default=4)
     ap.add_argument("--policy", default="refine", choices=["refine", "rerun"])
     ap.add_argument("--autotune", action="store_true",
@@ -216,13 +218,14 @@ def main(argv=None):
     res = run_batch(bases, target=args.target, precision=args.precision, workers=args.workers,
                     callback=show, verbose=args.verbose, max_samples=args.max_samples,
                     max_rounds=args.max_rounds, policy=args.policy, autotune=args.autotune,
+                    pilot_npts=args.pilot_npts,
                     tuning=tuning, on_tuned=keep_table)
     if args.autotune and args.tuning_file and "table" in saved:
         with open(args.tuning_file, "w") as f:
             f.write(AT.to_json(saved["table"]))
     s = summary(res, time.perf_counter() - t0)
     s.update(precision=args.precision, workers=args.workers, target=args.target,
-             policy=args.policy, autotune=args.autotune,
+             policy=args.policy, autotune=args.autotune, pilot_npts=args.pilot_npts,
              tuning=AT.to_json(tuning) if tuning else None)
     print(json.dumps(s), flush=True)
     return 0

@@ -69,7 +69,8 @@ class FakeBasis:
 def test_policy_reaches_target_in_two_runs():
     # pilot 40000 samples -> 2e-6; the second run is sized for 1e-6 x sqrt(1/1.2)
     eng = FakeEngine(s0=2e-6 * np.sqrt(40000))
-    r = B.run_molecule(eng, FakeBasis(), target=1e-6, precision="fp32", seed=7)
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6, precision="fp32", seed=7,
+                       pilot_npts=4000)
     assert r["rounds"] == 2 and r["reached"]
     assert [c[4] for c in eng.calls] == [0, 1]          # one Philox stream per run
     assert all(c[3] == 7 for c in eng.calls)
@@ -82,7 +83,8 @@ def test_policy_reaches_target_in_two_runs():
 
 def test_policy_stops_at_the_sample_budget():
     eng = FakeEngine(s0=1e-3 * np.sqrt(40000))          # hopeless: 1e-3 after the pilot
-    r = B.run_molecule(eng, FakeBasis(), target=1e-6, max_samples=1_000_000)
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6, max_samples=1_000_000,
+                       pilot_npts=4000)
     assert not r["reached"]
     assert r["samples"] <= 1_000_000
     assert r["rounds"] == 2                              # pilot + what the budget allows
@@ -91,8 +93,16 @@ def test_policy_stops_at_the_sample_budget():
 
 def test_policy_skips_production_when_the_pilot_suffices():
     eng = FakeEngine(s0=1e-7 * np.sqrt(40000))
-    r = B.run_molecule(eng, FakeBasis(), target=1e-6)
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6, pilot_npts=4000)
     assert r["rounds"] == 1 and r["reached"] and r["samples"] == 40000
+
+
+def test_default_pilot_is_ten_blocks_of_1000():
+    # the default pilot gives 10 x 1000 samples; 1e-6 after it needs no second run
+    eng = FakeEngine(s0=1e-6 * np.sqrt(10000))
+    r = B.run_molecule(eng, FakeBasis(), target=1e-6)
+    assert eng.calls[0][1] == 1000
+    assert r["rounds"] == 1 and r["reached"] and r["samples"] == 10000
 
 
 def test_batch_rejects_several_workers():
