@@ -238,6 +238,19 @@ __device__ inline void boys_rt(int L, double T, double *F)
     }
 }
 
+// 1/sqrt(x) for a positive normal x (an exponent sum p + q in the shell
+// kernels): the hardware estimate (about 5e-8 relative) and two Newton
+// steps y' = y + y (1 - x y^2) / 2, a few ulp; it replaces a correctly
+// rounded division and square root (1/s and sqrt(1/s)) in the prefactor.
+__device__ inline double rsqrt_pos(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-(x * y), y, 1.0);
+    y = fma(y * 0.5, e, y);
+    e = fma(-(x * y), y, 1.0);
+    return fma(y * 0.5, e, y);
+}
+
 // ------------------------------------------------------------------ R table
 // In place, levels n = L..0, totals s descending inside a level so that the
 // level-(n+1) entries a value needs are still intact:

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                 const double p = U[0], ip = U[1];
                 const double X = U[2] - Qx, Y = U[3] - Qy, Z = U[4] - Qz;
                 const double s = p + q;
-                const double is = 1.0 / s;
+                const double rs = rsqrt_pos(s), is = rs * rs;
                 const double alpha = (p * q) * is;
                 const double T = alpha * ((X * X + Y * Y) + Z * Z);
                 double F[L + 1];
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64) void k_eri_shell(PP pp, SPD sd,
                     boys<L>(T, F);
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
-                const double f = (U[5] * ck) * (((kPi52 * ip) * iq) * sqrt(is));
+                const double f = (U[5] * ck) * (((kPi52 * ip) * iq) * rs);
                 double Xv[CLB][HU];
                 if constexpr (SP) {
                     // HU independent FMA chains per lane component
@@ -484,7 +484,7 @@ void k_eri_shell_red(PP pp, SPD sd,
                 }
                 const double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
                 const double s = p + q;
-                const double is = 1.0 / s;
+                const double rs = rsqrt_pos(s), is = rs * rs;
                 const double alpha = (p * q) * is;
                 const double T = alpha * ((X * X + Y * Y) + Z * Z);
                 double F[L + 1];
@@ -494,7 +494,7 @@ void k_eri_shell_red(PP pp, SPD sd,
                     boys<L>(T, F);
                 double R[HR];
                 hermite_R<L>(alpha, X, Y, Z, F, R);
-                const double f = (cb * ck) * (((kPi52 * ip) * iq) * sqrt(is));
+                const double f = (cb * ck) * (((kPi52 * ip) * iq) * rs);
                 // the lane E (dense: with the prefactor and the Hermite sign
                 // folded in here; SP: lane_contract applies both)
                 double El[CLB][NZ];
