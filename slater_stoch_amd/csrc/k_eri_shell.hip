@@ -359,12 +359,13 @@ constexpr int pow2ceil(int x)
 
 // Reduction-mode batch size per class compiled for at least two waves per
 // SIMD (the register allocator then fits 256 registers, spilling if it
-// must); 0: none.  (2, 1) with sparse passes of 3 needs 260 registers
-// otherwise; at two waves it spills three (554 against 595 ms for the
-// dense passes at 254 registers, 908 ms at one wave).
+// must); 0: none.  (2, 1) with sparse passes of 3 needed it while its Boys
+// function was the erf form (260 registers otherwise; 554 against 908 ms);
+// with the table (two reduction rounds, shell_red_rounds) it takes 202
+// registers by itself, and no class uses it now.
 constexpr int kShellTwoWaves[5][5] = {{0, 0, 0, 0, 0},
                                       {0, 0, 0, 0, 0},
-                                      {0, 3, 0, 0, 0},
+                                      {0, 0, 0, 0, 0},
                                       {0, 0, 0, 0, 0},
                                       {0, 0, 0, 0, 0}};
 constexpr int shell_waves(int LU, int LL, int B) { return kShellTwoWaves[LU][LL] == B ? 2 : 1; }
@@ -376,7 +377,7 @@ constexpr int shell_waves(int LU, int LL, int B) { return kShellTwoWaves[LU][LL]
 // then contracts the sum with the E of its own uniform component u.  One LDS
 // reduction per (a, pass).
 // PF: lane data of the next chunk loaded one chunk ahead (kShellPrefetch)
-template <int LU, int LL, int CLB, bool PF, bool TB, bool SP>
+template <int LU, int LL, int CLB, bool PF, bool TB, bool SP, int NCH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(shell_waves(LU, LL, CLB))))
 void k_eri_shell_red(PP pp, SPD sd,
                                                       const ShellItem *__restrict__ items,
@@ -387,7 +388,9 @@ void k_eri_shell_red(PP pp, SPD sd,
     constexpr int HU = hcount(LU), L = LU + LL, HR = hcount(L);
     constexpr int NZ = SP ? kHP.nzmax[LL] : hcount(LL); // lane E per component
     constexpr int NR = CLB * HU; // rows summed over the wave
-    constexpr int S = NR | 1;    // odd lane stride of the transpose buffer
+    constexpr int RC = NR / NCH; // rows per transpose round
+    static_assert(NR % NCH == 0, "whole transpose rounds");
+    constexpr int S = RC | 1;    // odd lane stride of the transpose buffer
     __shared__ double lds[64 * S + NR];
     __shared__ double lbt[TB ? kBoysSlice : 1]; // Boys Taylor table, this L
     if constexpr (TB) boys_table_load<L>(pp.btab, lbt);
@@ -544,30 +547,38 @@ void k_eri_shell_red(PP pp, SPD sd,
             double ev[HU];
 #pragma unroll
             for (int hu = 0; hu < HU; hu++) ev[hu] = ea[(int64_t)hu * ncu];
+            // row r = j HU + hu summed over the 64 lanes into lds[64 S + r],
+            // RC rows per round through the transpose buffer (NCH rounds
+            // keep the buffer small enough for the Boys table beside it)
 #pragma unroll
-            for (int j = 0; j < CLB; j++)
+            for (int ch = 0; ch < NCH; ch++) {
+                if (ch > 0) __syncthreads(); // the previous round's reads are done
 #pragma unroll
-                for (int hu = 0; hu < HU; hu++) lds[lane * S + j * HU + hu] = Xa[j][hu];
-            __syncthreads();
-            if constexpr (NR <= 64) {
-                // P lanes per segment, 64 / P segments of each row
-                constexpr int P = pow2ceil(NR), G = 64 / P, SEG = 64 / G;
-                const int r = lane & (P - 1), k = lane / P;
-                double t4[4] = {0.0, 0.0, 0.0, 0.0};
-                if (r < NR) {
-#pragma unroll
-                    for (int i = 0; i < SEG; i++) t4[i & 3] += lds[(k * SEG + i) * S + r];
+                for (int rr = 0; rr < RC; rr++) {
+                    const int r = ch * RC + rr;
+                    lds[lane * S + rr] = Xa[r / HU][r % HU];
                 }
-                double v = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-#pragma unroll
-                for (int o = P; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
-                if (k == 0 && r < NR) lds[64 * S + r] = v;
-            } else {
-                for (int r = lane; r < NR; r += 64) {
+                __syncthreads();
+                if constexpr (RC <= 64) {
+                    // P lanes per segment, 64 / P segments of each row
+                    constexpr int P = pow2ceil(RC), G = 64 / P, SEG = 64 / G;
+                    const int r = lane & (P - 1), k = lane / P;
                     double t4[4] = {0.0, 0.0, 0.0, 0.0};
+                    if (r < RC) {
 #pragma unroll
-                    for (int i = 0; i < 64; i++) t4[i & 3] += lds[i * S + r];
-                    lds[64 * S + r] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                        for (int i = 0; i < SEG; i++) t4[i & 3] += lds[(k * SEG + i) * S + r];
+                    }
+                    double v = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+#pragma unroll
+                    for (int o = P; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+                    if (k == 0 && r < RC) lds[64 * S + ch * RC + r] = v;
+                } else {
+                    for (int r = lane; r < RC; r += 64) {
+                        double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int i = 0; i < 64; i++) t4[i & 3] += lds[i * S + r];
+                        lds[64 * S + ch * RC + r] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                    }
                 }
             }
             __syncthreads();
@@ -657,16 +668,31 @@ constexpr bool shell_prefetch(int LU, int LL, int B)
 // all whose LDS then still allows two workgroups of one wave per SIMD (20 KB
 // each), and the reduction kernels of order 4 and up, which run one wave
 // per SIMD anyway, up to 40 KB (four per CU).
-constexpr int shell_lds_bytes(int LU, int B, bool red)
+constexpr int shell_lds_bytes(int LU, int B, bool red, int nch = 1)
 {
-    const int HU = hcount(LU), NR = B * HU, S = NR | 1;
+    const int HU = hcount(LU), NR = B * HU, S = (NR / nch) | 1;
     return red ? (64 * S + NR) * 8 : kDirectMaxU * B * 64 * 8;
+}
+constexpr bool shell_table_fits(int LU, int LL, int B, bool red, int nch)
+{
+    const int bytes = shell_lds_bytes(LU, B, red, nch) + kBoysSlice * 8;
+    return LU + LL <= kBoysLmax &&
+           (bytes <= 20480 || (red && LU + LL >= 4 && bytes <= 40960));
+}
+// Reduction instances whose transpose buffer would leave no room for the
+// Boys table sum their rows in two rounds of half the rows instead, where
+// that makes it fit ((2, 1) with passes of 3, (4, x) with passes of 2).
+constexpr int shell_red_rounds(int LU, int LL, int B)
+{
+    const int NR = B * hcount(LU);
+    return !shell_table_fits(LU, LL, B, true, 1) && NR % 2 == 0 &&
+                   shell_table_fits(LU, LL, B, true, 2)
+               ? 2
+               : 1;
 }
 constexpr bool shell_boys_table(int LU, int LL, int B, bool red)
 {
-    const int bytes = shell_lds_bytes(LU, B, red) + kBoysSlice * 8;
-    return LU + LL <= kBoysLmax &&
-           (bytes <= 20480 || (red && LU + LL >= 4 && bytes <= 40960));
+    return shell_table_fits(LU, LL, B, red, red ? shell_red_rounds(LU, LL, B) : 1);
 }
 // Direct-mode classes whose uniform data are loaded one iteration ahead into
 // vector registers (uniform side of order 0 or 1: at most 18 doubles): the
@@ -708,7 +734,8 @@ void launch_shell_b(bool red, unsigned grid, hipStream_t st, const PP &pp, const
     if (red)
         hipLaunchKernelGGL((k_eri_shell_red<LU, LL, B, shell_prefetch(LU, LL, B),
                                             shell_boys_table(LU, LL, B, true),
-                                            shell_sparse(LU, LL, true)>),
+                                            shell_sparse(LU, LL, true),
+                                            shell_red_rounds(LU, LL, B)>),
                            dim3(grid), dim3(64), 0, st, pp, sd, items, sd.pu, sd.eu2, part);
     else
         hipLaunchKernelGGL((k_eri_shell<LU, LL, B, shell_prefetch_u(LU, LL, B),
