@@ -256,8 +256,10 @@ int zv_davidson_lowest(zv_context *ctx, int nev, double tol, int max_iter, int p
  * a -- b / N" per randomized pass, src/svd_clean.F90:153-154, and the
  * largest/smallest singular value of the pass, :285); level 2 adds one line
  * per finished zv_mc_blocks block (the call then waits for each block, so
- * the line reports work done); level 0 (the default) prints nothing.  No
- * counterpart in the reference, whose prints cannot be switched off. */
+ * the line reports work done); level 0 (the default) prints nothing.  From
+ * level 1 on, zv_gauss_eri also writes the host time of its Schwarz
+ * phases' planning to stderr (lines starting "[zvmc]").  No counterpart
+ * in the reference, whose prints cannot be switched off. */
 int zv_set_verbose(zv_context *ctx, int level);
 
 /* Tuning knobs (0 = the library's automatic choice, the default).  No
